@@ -1,0 +1,262 @@
+/*
+ * p3dfft.h -- public C++ API of the MI355X-native P3DFFT++ transform path.
+ *
+ * Source-compatible with the reference's include/p3dfft.h for everything a
+ * program on a 1x1 processor grid uses (namespace p3dfft, setup/cleanup,
+ * the type-ID globals, trans_type3D, ProcGrid, DataGrid, transform3D,
+ * transplan, compute_deriv, inv_mo).  The classes are implemented over the
+ * HIP engine in libp3dfft3_hip.so; their private layout is not the
+ * reference's.  Arrays passed to exec may live in device memory (used in
+ * place) or in host memory (staged through device buffers).
+ *
+ * Reference interfaces replaced (include/p3dfft.h of P3DFFT++ 3.0.0):
+ *   P3DFFT_* C type-ID globals  :105-123
+ *   namespace type-ID globals   :155-160
+ *   mycomplex/complex_double    :194-195
+ *   setup/cleanup               :208-209
+ *   rel_change/inv_mo           :215-216
+ *   gen_trans_type              :235-272
+ *   trans_type1D                :482-522
+ *   stage                       :529-547
+ *   transplan                   :581-623
+ *   ProcGrid                    :653-700
+ *   DataGrid                    :702-755
+ *   trans_type3D                :832-849
+ *   find_order                  :851
+ *   gen_transform3D             :869-875
+ *   transform3D                 :877-896
+ *   compute_deriv               :917
+ * MPIplan / trans_MPIplan (:549-577, :625-651) are multi-rank exchange
+ * stages, never built on a 1x1 grid, and are not provided.
+ */
+#ifndef P3DFFT_H
+#define P3DFFT_H
+
+#include "mpi.h"
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+
+#define Type3D int
+#define Type1D int
+#define Plan3D int
+#define Plan1D int
+
+#ifdef __cplusplus
+
+#include <complex>
+#include <iostream>
+#include <typeinfo>
+#include <vector>
+
+namespace p3dfft {
+
+using namespace std;
+
+typedef complex<float> mycomplex;
+typedef complex<double> complex_double;
+
+static const int REAL = 1;
+static const int COMPLEX = 2;
+
+static const int TRANS_ONLY = 1;
+static const int MPI_ONLY = 2;
+static const int TRANSMPI = 3;
+
+extern int EMPTY_TYPE_SINGLE, EMPTY_TYPE_DOUBLE, EMPTY_TYPE_SINGLE_COMPLEX, EMPTY_TYPE_DOUBLE_COMPLEX;
+extern int R2CFFT_S, R2CFFT_D, C2RFFT_S, C2RFFT_D, CFFT_FORWARD_S, CFFT_FORWARD_D, CFFT_BACKWARD_S, CFFT_BACKWARD_D;
+extern int DCT1_REAL_S, DCT1_REAL_D, DST1_REAL_S, DST1_REAL_D, DCT1_COMPLEX_S, DCT1_COMPLEX_D, DST1_COMPLEX_S, DST1_COMPLEX_D;
+extern int DCT2_REAL_S, DCT2_REAL_D, DST2_REAL_S, DST2_REAL_D, DCT2_COMPLEX_S, DCT2_COMPLEX_D, DST2_COMPLEX_S, DST2_COMPLEX_D;
+extern int DCT3_REAL_S, DCT3_REAL_D, DST3_REAL_S, DST3_REAL_D, DCT3_COMPLEX_S, DCT3_COMPLEX_D, DST3_COMPLEX_S, DST3_COMPLEX_D;
+extern int DCT4_REAL_S, DCT4_REAL_D, DST4_REAL_S, DST4_REAL_D, DCT4_COMPLEX_S, DCT4_COMPLEX_D, DST4_COMPLEX_S, DST4_COMPLEX_D;
+
+void setup();
+void cleanup();
+
+const int CACHEPAD = 32768;
+const int CACHE_BL = 32768;
+const int VECBLOCK = 128;
+
+void rel_change(int *, int *, int *);
+void inv_mo(int mo[3], int imo[3]);
+int arcmp(int *A, int *B, int N);
+
+class gen_trans_type {
+ public:
+  int isign;
+  bool is_set, is_empty;
+  int dt1, dt2;  // 1 = real, 2 = complex, before and after
+  int prec;      // 4 or 8
+  int cls;       // engine transform class (empty, r2c, c2r, c2c, r2r)
+  int r2r_kind;  // FFTW r2r kind for the r2r class
+  const char *name;
+  gen_trans_type(int isign_ = 0)
+      : isign(isign_), is_set(true), is_empty(false), dt1(0), dt2(0), prec(0), cls(0),
+        r2r_kind(0), name("") {}
+  gen_trans_type(int dt1_, int dt2_, int prec_, int isign_ = 0)
+      : isign(isign_), is_set(true), is_empty(false), dt1(dt1_), dt2(dt2_), prec(prec_),
+        cls(0), r2r_kind(0), name("") {}
+  virtual ~gen_trans_type() {}
+  inline bool operator==(const gen_trans_type &rhs) const {
+    return rhs.isign == isign && rhs.is_set == is_set && rhs.dt1 == dt1 && rhs.dt2 == dt2;
+  }
+};
+
+template <class Type1, class Type2> class trans_type1D : public gen_trans_type {
+  int ID;
+ public:
+  trans_type1D(const char *name_, int cls_, int r2r_kind_, int isign_, bool empty_);
+  inline int getID() { return ID; }
+  void setID(int id) { ID = id; }
+};
+
+class ProcGrid {
+ public:
+  int taskid, numtasks;
+  int nd;
+  int ProcDims[3];
+  int grid_id_cart[3];
+  MPI_Comm mpi_comm_glob;
+  MPI_Comm mpi_comm_cart;
+  MPI_Comm mpicomm[3];
+
+  ProcGrid(int procdims[3], MPI_Comm mpi_comm_init);
+  ProcGrid(const ProcGrid &rhs);
+  ~ProcGrid();
+  bool operator==(const ProcGrid &P) const;
+};
+
+class DataGrid {
+ public:
+  int nd;
+  int Gdims[3];
+  int dim_conj_sym;
+  int MemOrder[3];
+  int Ldims[3];
+  ProcGrid *Pgrid;
+  int Pdims[3];
+  int Dmap[3];
+  int L[3];
+  int D[3];
+  int GlobStart[3];
+  int grid_id[3];
+  bool is_set;
+
+  inline bool IsLocal(int dim) {
+    if (dim < 0 || dim > 2) return false;
+    return Pdims[dim] == 1;
+  }
+  DataGrid(int *gdims_, int dim_conj_sym_, ProcGrid *pgrid, int *dmap, int *mem_order);
+  DataGrid(const DataGrid &rhs);
+  DataGrid() : is_set(false) {}
+  ~DataGrid() {}
+  void set_gdims(int gdims[3]);
+  void get_gdims(int gdims[3]) { for (int i = 0; i < 3; i++) gdims[i] = Gdims[i]; }
+  void set_mo(int mo[3]) { for (int i = 0; i < 3; i++) MemOrder[i] = mo[i]; }
+
+ private:
+  void InitPencil();
+};
+
+class trans_type3D {
+ public:
+  char *name;
+  int prec;
+  bool is_set;
+  int types[3];
+  trans_type3D(gen_trans_type *types_[3]);
+  trans_type3D(int types_IDs[3]);
+  trans_type3D(const trans_type3D &rhs);
+  trans_type3D &operator=(const trans_type3D &rhs);
+  ~trans_type3D();
+};
+
+class stage {
+ public:
+  int stage_prec;
+  int dt1, dt2;
+  int dims1[3], dims2[3];
+  stage *next;
+  int kind;
+  stage() : next(NULL), kind(TRANS_ONLY) {}
+  virtual ~stage() {}
+};
+
+/* Opaque engine plan (defined inside the library). */
+struct engine_plan;
+
+template <class Type1, class Type2> class transplan : public stage {
+ public:
+  bool is_empty;
+  int trans_dim;
+  int mo1[3], mo2[3];
+  bool is_set;
+  int prec;
+  int N, m, isign;
+  gen_trans_type *trans_type;
+  DataGrid *grid1, *grid2;
+
+  transplan(const DataGrid &gr1, const DataGrid &gr2, const gen_trans_type *type, int d);
+  transplan(const DataGrid &gr1, const DataGrid &gr2, int type_ID, int d);
+  ~transplan();
+  void exec(char *in, char *out, bool OW = false);
+  void exec_deriv(char *in, char *out, bool OW = false);
+
+ private:
+  void init_tr(const DataGrid &gr1, const DataGrid &gr2, const gen_trans_type *type, int d);
+  engine_plan *eng;
+  transplan(const transplan &);
+  transplan &operator=(const transplan &);
+};
+
+class gen_transform3D {
+ public:
+  int prec;
+  int dt1, dt2;
+  bool OW;
+  gen_transform3D() : prec(0), dt1(0), dt2(0), OW(false), eng(NULL) {}
+  virtual ~gen_transform3D() {}
+  /* Engine handle, for the asynchronous and introspection extensions. */
+  engine_plan *engine() const { return eng; }
+
+ protected:
+  engine_plan *eng;
+};
+
+template <class Type1, class Type2> class transform3D : public gen_transform3D {
+  bool is_set;
+  DataGrid *grid1, *grid2;
+  transform3D(const transform3D &);
+  transform3D &operator=(const transform3D &);
+
+ public:
+  void exec(Type1 *in, Type2 *out, bool OW = false);
+  void exec_deriv(Type1 *in, Type2 *out, int idir, bool OW = false);
+  transform3D(const DataGrid &grid1_, const DataGrid &grid2_, const trans_type3D *type);
+  ~transform3D();
+};
+
+bool find_order(int L[3], const trans_type3D *tp, const DataGrid *gr1, const DataGrid *gr2,
+                bool *return_steps);
+
+template <class Type> void compute_deriv(Type *in, Type *out, DataGrid *gr, int idir);
+
+extern vector<gen_trans_type *> types1D;
+extern vector<trans_type3D> types3D;
+extern vector<gen_transform3D *> stored_trans3D;
+extern vector<stage *> stored_trans1D;
+extern vector<ProcGrid *> stored_proc_grids;
+extern vector<DataGrid *> stored_data_grids;
+
+}  // namespace p3dfft
+
+extern "C" {
+#endif /* __cplusplus */
+
+#include "Cwrap.h"
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* P3DFFT_H */

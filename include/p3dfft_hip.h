@@ -1,0 +1,50 @@
+/*
+ * p3dfft_hip.h -- extensions of libp3dfft3_hip.so beyond the reference ABI.
+ * None of these has a counterpart in the reference's Cwrap.h; they serve
+ * callers that keep arrays on the GPU and drive their own streams
+ * (benchmarks, the Poisson service, profilers).
+ *
+ * Stream arguments are hipStream_t passed as void* (NULL = the library's
+ * stream).  Array pointers passed to the *_async functions must be device
+ * pointers; those functions return before the work completes.
+ */
+#ifndef P3DFFT_HIP_H
+#define P3DFFT_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Last error recorded by the library: returns 0 if none, else nonzero, and
+ * copies the message into buf (NUL-terminated, at most len bytes). */
+int p3dfft_hip_last_error(char *buf, int len);
+void p3dfft_hip_clear_error(void);
+
+/* Asynchronous exec of a 3D plan on a stream (device pointers only).
+ * idir < 0: no derivative.  Returns 0 on success. */
+int p3dfft_exec_3Dtrans_double_async(int plan, double *in, double *out, int idir, int OW, void *stream);
+int p3dfft_exec_3Dtrans_single_async(int plan, float *in, float *out, int idir, int OW, void *stream);
+
+/* The library's own stream (created on first use). */
+void *p3dfft_hip_stream(void);
+
+/* Introspection of a 3D plan: number of passes, and a text description. */
+int p3dfft_hip_plan_npasses(int plan);
+int p3dfft_hip_plan_describe(int plan, char *buf, int len);
+/* Bytes read and written by pass i of a 3D plan (for roofline accounting). */
+int p3dfft_hip_pass_bytes(int plan, int i, long long *bytes_in, long long *bytes_out);
+
+/* Per-pass timing of a 3D plan with HIP events recorded on the exec stream
+ * around every pass launch.  p3dfft_hip_profile(plan, 1) starts (and
+ * resets) collection; p3dfft_hip_profile_read waits for the recorded events
+ * and writes the average duration in ms of each pass (up to cap entries)
+ * and the summed duration of all recorded launches.  Returns the number of
+ * recorded executions, or -1. */
+int p3dfft_hip_profile(int plan, int enable);
+int p3dfft_hip_profile_read(int plan, double *avg_ms, int cap, double *total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* P3DFFT_HIP_H */

@@ -1,0 +1,290 @@
+// Engine of libp3dfft3_hip.so: uploads the constant tables of a planned pass
+// chain, assigns buffers and launches the passes.
+//
+// A transform along one axis is one pass: the kernel reads every element of
+// its input once and writes every element of its output once, permuting the
+// memory order on the way when the next pass or the caller wants another
+// layout.  The reference runs each stage as a local transpose loop plus an
+// FFTW call, i.e. two memory passes and a temporary allocation per exec
+// (build/exec.C:101-223, 522-622, 737-2266); here the intermediate layouts
+// are chosen at plan time so that both sides of each pass stay coalesced,
+// and all temporaries are owned by the plan and reused across calls.
+#include "engine.h"
+
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+
+extern "C" int p3e_launch_pass(const p3e::PassArgs *a, int prec, size_t lds_bytes, int threads,
+                               unsigned int nblocks, hipStream_t st);
+extern "C" int p3e_launch_deriv(const void *in, void *out, long long n, int s0, int s1, int ldir, int g,
+                                int mid, int gstart, int prec, hipStream_t st);
+
+namespace p3e {
+
+#define HIPCHK(x)                                                \
+  do {                                                           \
+    hipError_t e_ = (x);                                         \
+    if (e_ != hipSuccess) {                                      \
+      set_error("%s failed: %s", #x, hipGetErrorString(e_));     \
+      return -1;                                                 \
+    }                                                            \
+  } while (0)
+
+static hipStream_t g_stream = nullptr;
+
+hipStream_t default_stream() {
+  if (!g_stream) {
+    if (hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking) != hipSuccess) g_stream = nullptr;
+  }
+  return g_stream;
+}
+
+void release_all() {
+  if (g_stream) {
+    (void)hipStreamSynchronize(g_stream);
+    (void)hipStreamDestroy(g_stream);
+    g_stream = nullptr;
+  }
+}
+
+template <typename T> static void *upload(const std::vector<cld> &v, std::vector<void *> &owned) {
+  if (v.empty()) return nullptr;
+  std::vector<T> h(2 * v.size());
+  for (size_t i = 0; i < v.size(); i++) {
+    h[2 * i] = (T)v[i].real();
+    h[2 * i + 1] = (T)v[i].imag();
+  }
+  void *d = nullptr;
+  if (hipMalloc(&d, h.size() * sizeof(T)) != hipSuccess) return nullptr;
+  if (hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(d);
+    return nullptr;
+  }
+  owned.push_back(d);
+  return d;
+}
+
+static bool upload_tables(int prec, Pass &p) {
+  PassArgs &a = p.d.args;
+  const std::vector<cld> *src[5] = {&p.d.tw, &p.d.chirp, &p.d.chirp_hat, &p.d.pre, &p.d.post};
+  const void **dst[5] = {&a.tw, &a.chirp, &a.chirp_hat, &a.pre, &a.post};
+  for (int i = 0; i < 5; i++) {
+    if (src[i]->empty()) {
+      *dst[i] = nullptr;
+      continue;
+    }
+    void *d = (prec == 8) ? upload<double>(*src[i], p.tables) : upload<float>(*src[i], p.tables);
+    if (!d) {
+      set_error("hipMalloc/hipMemcpy failed for a constant table");
+      return false;
+    }
+    *dst[i] = d;
+  }
+  return true;
+}
+
+static void free_tables(Pass &p) {
+  for (void *t : p.tables) (void)hipFree(t);
+  p.tables.clear();
+}
+
+Plan *plan_create(int prec, const OpSpec ops[3], const int order[3], const Layout &in, const Layout &out) {
+  PassChain chain;
+  if (!plan_chain(prec, ops, order, in, out, chain)) return nullptr;
+  Plan *p = new Plan();
+  p->prec = prec;
+  p->lin = in;
+  p->lout = out;
+  p->in_bytes = layout_elems(in) * elem_bytes(prec, in.dt);
+  p->out_bytes = layout_elems(out) * elem_bytes(prec, out.dt);
+  p->scratch[0] = p->scratch[1] = nullptr;
+  p->scratch_cap[0] = p->scratch_cap[1] = 0;
+  p->stage_in = p->stage_out = nullptr;
+  p->stage_in_cap = p->stage_out_cap = 0;
+  p->copy_only = chain.copy_only;
+  p->desc = chain.desc;
+  p->prof = false;
+  p->ev_used = 0;
+  for (int a = 0; a < 3; a++) p->pass_of_axis[a] = chain.pass_of_axis[a];
+  for (PassDesc &d : chain.passes) {
+    Pass ps;
+    ps.d = d;
+    if (!upload_tables(prec, ps)) {
+      free_tables(ps);
+      plan_destroy(p);
+      return nullptr;
+    }
+    p->passes.push_back(ps);
+  }
+  return p;
+}
+
+void plan_destroy(Plan *p) {
+  if (!p) return;
+  for (hipEvent_t e : p->ev_pool) (void)hipEventDestroy(e);
+  for (Pass &ps : p->passes) free_tables(ps);
+  for (int i = 0; i < 2; i++)
+    if (p->scratch[i]) (void)hipFree(p->scratch[i]);
+  if (p->stage_in) (void)hipFree(p->stage_in);
+  if (p->stage_out) (void)hipFree(p->stage_out);
+  delete p;
+}
+
+bool is_device_ptr(const void *ptr) {
+  if (!ptr) return false;
+  hipPointerAttribute_t attr;
+  hipError_t e = hipPointerGetAttributes(&attr, ptr);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+static int ensure(void **buf, size_t *cap, size_t need) {
+  if (*buf && *cap >= need) return 0;
+  if (*buf) {
+    (void)hipFree(*buf);
+    *buf = nullptr;
+    *cap = 0;
+  }
+  if (need == 0) return 0;
+  HIPCHK(hipMalloc(buf, need));
+  *cap = need;
+  return 0;
+}
+
+int plan_exec(Plan *p, const void *in, void *out, int idir, bool OW, hipStream_t st, bool sync) {
+  (void)OW;  // the user input is never written, so OW only relaxes checks
+  if (!p) {
+    set_error("exec on an invalid plan");
+    return -1;
+  }
+  if (!st) st = default_stream();
+  const void *din = in;
+  void *dout = out;
+  const bool in_host = !is_device_ptr(in);
+  const bool out_host = !is_device_ptr(out);
+  if (in_host) {
+    if (ensure(&p->stage_in, &p->stage_in_cap, std::max(p->in_bytes, (size_t)1))) return -1;
+    HIPCHK(hipMemcpyAsync(p->stage_in, in, p->in_bytes, hipMemcpyHostToDevice, st));
+    din = p->stage_in;
+  }
+  if (out_host) {
+    if (ensure(&p->stage_out, &p->stage_out_cap, std::max(p->out_bytes, (size_t)1))) return -1;
+    dout = p->stage_out;
+  }
+  const bool alias = (din == (const void *)dout);
+  if (p->copy_only) {
+    if (!alias) HIPCHK(hipMemcpyAsync(dout, din, p->out_bytes, hipMemcpyDeviceToDevice, st));
+  } else {
+    std::vector<PassDesc> descs;
+    for (Pass &ps : p->passes) descs.push_back(ps.d);
+    int dst[8];
+    size_t need[2];
+    bool copy_back = false;
+    assign_buffers(descs, p->out_bytes, alias, dst, need, &copy_back);
+    for (int s = 0; s < 2; s++)
+      if (need[s] && ensure(&p->scratch[s], &p->scratch_cap[s], need[s])) return -1;
+    const void *src = din;
+    for (size_t i = 0; i < p->passes.size(); i++) {
+      Pass &ps = p->passes[i];
+      void *d = nullptr;
+      switch (dst[i]) {
+        case BUF_OUT: d = dout; break;
+        case BUF_S0: d = p->scratch[0]; break;
+        default: d = p->scratch[1]; break;
+      }
+      PassArgs a = ps.d.args;
+      a.in = src;
+      a.out = d;
+      a.deriv = (idir >= 0 && idir == ps.d.axis && ps.d.deriv_capable) ? 1 : 0;
+      hipEvent_t e0 = nullptr, e1 = nullptr;
+      if (p->prof) {
+        while (p->ev_pool.size() < p->ev_used + 2) {
+          hipEvent_t e;
+          HIPCHK(hipEventCreate(&e));
+          p->ev_pool.push_back(e);
+        }
+        e0 = p->ev_pool[p->ev_used];
+        e1 = p->ev_pool[p->ev_used + 1];
+        p->ev_used += 2;
+        p->ev_pass.push_back((int)i);
+        HIPCHK(hipEventRecord(e0, st));
+      }
+      int rc = p3e_launch_pass(&a, p->prec, ps.d.lds_bytes, ps.d.threads, ps.d.nblocks, st);
+      if (p->prof) HIPCHK(hipEventRecord(e1, st));
+      if (rc) {
+        set_error("pass kernel launch failed: %s", hipGetErrorString((hipError_t)rc));
+        return -1;
+      }
+      src = d;
+    }
+    if (copy_back) HIPCHK(hipMemcpyAsync(dout, src, p->out_bytes, hipMemcpyDeviceToDevice, st));
+  }
+  if (out_host) HIPCHK(hipMemcpyAsync(out, dout, p->out_bytes, hipMemcpyDeviceToHost, st));
+  if (sync || in_host || out_host) HIPCHK(hipStreamSynchronize(st));
+  return 0;
+}
+
+int plan_profile(Plan *p, bool enable) {
+  if (!p) return -1;
+  p->prof = enable;
+  p->ev_used = 0;
+  p->ev_pass.clear();
+  return 0;
+}
+
+int plan_profile_read(Plan *p, double *avg_ms, int cap, double *total_ms) {
+  if (!p) return -1;
+  const int np = (int)p->passes.size();
+  std::vector<double> sum(np, 0.0);
+  std::vector<int> cnt(np, 0);
+  double tot = 0;
+  for (size_t k = 0; k < p->ev_pass.size(); k++) {
+    float ms = 0;
+    HIPCHK(hipEventSynchronize(p->ev_pool[2 * k + 1]));
+    HIPCHK(hipEventElapsedTime(&ms, p->ev_pool[2 * k], p->ev_pool[2 * k + 1]));
+    sum[p->ev_pass[k]] += ms;
+    cnt[p->ev_pass[k]]++;
+    tot += ms;
+  }
+  for (int i = 0; i < np && i < cap; i++) avg_ms[i] = cnt[i] ? sum[i] / cnt[i] : 0.0;
+  if (total_ms) *total_ms = tot;
+  return np ? cnt[0] : 0;
+}
+
+int deriv_exec(int prec, const void *in, void *out, const int sdims[3], int ldir, int g, int mid, int gstart,
+               hipStream_t st) {
+  if (!st) st = default_stream();
+  const long long n = (long long)sdims[0] * sdims[1] * sdims[2];
+  const size_t bytes = (size_t)n * 2 * prec;
+  const bool in_host = !is_device_ptr(in), out_host = !is_device_ptr(out);
+  const void *di = in;
+  void *dout = out;
+  void *tmp_in = nullptr, *tmp_out = nullptr;
+  if (in_host) {
+    HIPCHK(hipMalloc(&tmp_in, bytes));
+    HIPCHK(hipMemcpyAsync(tmp_in, in, bytes, hipMemcpyHostToDevice, st));
+    di = tmp_in;
+  }
+  if (out_host) {
+    if (in_host && in == out) tmp_out = tmp_in;
+    else HIPCHK(hipMalloc(&tmp_out, bytes));
+    dout = tmp_out;
+  }
+  int rc = p3e_launch_deriv(di, dout, n, sdims[0], sdims[1], ldir, g, mid, gstart, prec, st);
+  if (rc) {
+    set_error("derivative kernel launch failed: %s", hipGetErrorString((hipError_t)rc));
+    return -1;
+  }
+  if (out_host) HIPCHK(hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (tmp_in) (void)hipFree(tmp_in);
+  if (tmp_out && tmp_out != tmp_in) (void)hipFree(tmp_out);
+  return 0;
+}
+
+}  // namespace p3e

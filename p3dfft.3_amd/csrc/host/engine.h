@@ -1,0 +1,58 @@
+// Internal engine of libp3dfft3_hip.so: runs a planned chain of fused
+// reorder + 1D transform passes on a HIP stream.  Not a public header.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "pass_plan.h"
+
+namespace p3e {
+
+struct Pass {
+  PassDesc d;                  // geometry and host tables
+  std::vector<void *> tables;  // device copies owned by the pass
+};
+
+struct Plan {
+  int prec;  // 4 or 8
+  Layout lin, lout;
+  size_t in_bytes, out_bytes;
+  std::vector<Pass> passes;
+  int pass_of_axis[3];
+  bool copy_only;
+  void *scratch[2];
+  size_t scratch_cap[2];
+  void *stage_in, *stage_out;
+  size_t stage_in_cap, stage_out_cap;
+  std::string desc;
+  // per-pass timing with HIP events (off unless enabled)
+  bool prof;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<int> ev_pass;  // pass index of each recorded (start, stop) pair
+  size_t ev_used;
+};
+
+int plan_profile(Plan *p, bool enable);
+// Average duration of each pass over the launches recorded since enabling;
+// returns the number of recorded launches of pass 0.
+int plan_profile_read(Plan *p, double *avg_ms, int cap, double *total_ms);
+
+// ops[a]: operation on logical axis a; order: axes in execution order (R2C
+// first, C2R last).  Returns nullptr on error (message in last_error()).
+Plan *plan_create(int prec, const OpSpec ops[3], const int order[3], const Layout &in, const Layout &out);
+// idir: logical axis of the spectral derivative, or -1.  in/out may be host
+// or device pointers; host arrays are staged and imply a synchronous call.
+int plan_exec(Plan *p, const void *in, void *out, int idir, bool OW, hipStream_t st, bool sync);
+void plan_destroy(Plan *p);
+
+// Standalone derivative of a stored spectrum (compute_deriv semantics).
+int deriv_exec(int prec, const void *in, void *out, const int sdims[3], int ldir, int g, int mid, int gstart,
+               hipStream_t st);
+
+bool is_device_ptr(const void *ptr);
+hipStream_t default_stream();
+void release_all();
+
+}  // namespace p3e

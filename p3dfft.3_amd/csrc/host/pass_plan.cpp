@@ -1,0 +1,492 @@
+// Pass planning (plain C++).  See pass_plan.h.
+#include "pass_plan.h"
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+
+namespace p3e {
+
+// ------------------------------------------------------------------ errors
+static char g_err[1024];
+static int g_err_code = 0;
+
+void set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  g_err_code = 1;
+  fprintf(stderr, "p3dfft3_hip: %s\n", g_err);
+}
+const char *last_error() { return g_err; }
+int last_error_code() { return g_err_code; }
+void clear_error() {
+  g_err[0] = 0;
+  g_err_code = 0;
+}
+
+// ------------------------------------------------------------------ helpers
+size_t layout_elems(const Layout &l) { return (size_t)l.dims[0] * l.dims[1] * l.dims[2]; }
+size_t elem_bytes(int prec, int dt) { return (size_t)prec * dt; }
+
+static void strides_of(const Layout &l, long long st[3]) {
+  for (int a = 0; a < 3; a++) {
+    long long s = 1;
+    for (int b = 0; b < 3; b++)
+      if (l.mo[b] < l.mo[a]) s *= l.dims[b];
+    st[a] = s;
+  }
+}
+
+static int stride1_axis(const int mo[3]) {
+  for (int a = 0; a < 3; a++)
+    if (mo[a] == 0) return a;
+  return 0;
+}
+
+// Radix plan for length n: 8s (with one 16 or 4 absorbing the remainder) for
+// the power-of-two part, then odd primes up to 31, largest first.  Returns
+// false if a larger prime factor remains (the pass then uses Bluestein).
+static bool factor(int n, std::vector<int> &rad) {
+  rad.clear();
+  int m = n, p2 = 0;
+  while (m % 2 == 0) {
+    m /= 2;
+    p2++;
+  }
+  std::vector<int> twos;
+  while (p2 >= 3) {
+    twos.push_back(8);
+    p2 -= 3;
+  }
+  if (p2 == 2) twos.push_back(4);
+  if (p2 == 1) {
+    if (!twos.empty()) twos[0] = 16;
+    else twos.push_back(2);
+  }
+  std::vector<int> odd;
+  for (int q : {3, 5, 7, 11, 13, 17, 19, 23, 29, 31}) {
+    while (m % q == 0) {
+      odd.push_back(q);
+      m /= q;
+    }
+  }
+  if (m != 1) return false;
+  std::sort(odd.begin(), odd.end(), [](int a, int b) { return a > b; });
+  rad = twos;
+  rad.insert(rad.end(), odd.begin(), odd.end());
+  return (int)rad.size() <= P3E_MAX_RADIX_PASSES;
+}
+
+static int next_pow2(int n) {
+  int p = 1;
+  while (p < n) p <<= 1;
+  return p;
+}
+
+static const long double PI_L = 3.141592653589793238462643383279502884L;
+
+// exp(i * pi * num / den), num reduced modulo 2*den in integers first.
+static cld expi_pi(long long num, long long den) {
+  long long m = num % (2 * den);
+  if (m < 0) m += 2 * den;
+  long double a = PI_L * (long double)m / (long double)den;
+  return cld(cosl(a), sinl(a));
+}
+
+// In-place radix-2 DFT (sign -1) in long double, n a power of two.  Plan-time
+// only: it computes the spectrum of the Bluestein chirp, a constant table.
+static void host_fft_pow2(std::vector<cld> &a) {
+  const size_t n = a.size();
+  for (size_t i = 1, j = 0; i < n; i++) {
+    size_t bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) std::swap(a[i], a[j]);
+  }
+  for (size_t len = 2; len <= n; len <<= 1) {
+    for (size_t i = 0; i < n; i += len) {
+      for (size_t k = 0; k < len / 2; k++) {
+        cld w = expi_pi(-(long long)(2 * k), (long long)len);
+        cld u = a[i + k], v = a[i + k + len / 2] * w;
+        a[i + k] = u + v;
+        a[i + k + len / 2] = u - v;
+      }
+    }
+  }
+}
+
+static const char *cls_name(int c) {
+  switch (c) {
+    case CLS_EMPTY: return "copy";
+    case CLS_R2C: return "r2c";
+    case CLS_C2R: return "c2r";
+    case CLS_C2C: return "c2c";
+    case CLS_R2R: return "r2r";
+  }
+  return "?";
+}
+
+static const size_t LDS_BUDGET = 80 * 1024;
+static const size_t LDS_MAX = 160 * 1024;
+
+// ------------------------------------------------------------------ one pass
+bool describe_pass(int prec, int axis, const OpSpec &op, const Layout &lin, const Layout &lout, PassDesc &ps) {
+  PassArgs &a = ps.args;
+  memset(&a, 0, sizeof(a));
+  ps.axis = axis;
+  ps.lin = lin;
+  ps.lout = lout;
+  const int d = axis;
+  const int nin = lin.dims[d], nout = lout.dims[d];
+  a.cls = op.cls;
+  a.r2r_kind = op.r2r_kind;
+  a.in_cplx = lin.dt == 2;
+  a.out_cplx = lout.dt == 2;
+  a.nin = nin;
+  a.nout = nout;
+  for (int b = 0; b < 3; b++) {
+    if (lin.dims[b] < 1 || lout.dims[b] < 1) {
+      set_error("array dims must be positive");
+      return false;
+    }
+    if (b != d && lin.dims[b] != lout.dims[b]) {
+      set_error("pass along axis %d: dims of other axes differ", d);
+      return false;
+    }
+  }
+  int N = nin;
+  switch (op.cls) {
+    case CLS_EMPTY:
+    case CLS_C2C:
+    case CLS_R2R:
+      if (nin != nout) {
+        set_error("transform along axis %d: input length %d != output length %d", d, nin, nout);
+        return false;
+      }
+      break;
+    case CLS_R2C:
+      if (nout != nin / 2 + 1) {
+        set_error("R2C along axis %d: output length %d, expected %d", d, nout, nin / 2 + 1);
+        return false;
+      }
+      break;
+    case CLS_C2R:
+      N = nout;
+      if (nin != nout / 2 + 1) {
+        set_error("C2R along axis %d: input length %d, expected %d", d, nin, nout / 2 + 1);
+        return false;
+      }
+      break;
+  }
+  a.N = N;
+  int W = N, sign = -1;
+  switch (op.cls) {
+    case CLS_C2C: sign = op.sign; break;
+    case CLS_C2R: sign = +1; break;
+    case CLS_R2R:
+      switch (op.r2r_kind) {
+        case REDFT00:
+          if (N < 2) {
+            set_error("DCT-I needs at least 2 points (axis %d has %d)", d, N);
+            return false;
+          }
+          W = 2 * (N - 1);
+          break;
+        case RODFT00: W = 2 * (N + 1); break;
+        case REDFT10:
+        case RODFT10: W = 2 * N; break;
+        default: W = 2 * N; sign = +1; break;
+      }
+      break;
+    default: break;
+  }
+  a.W = W;
+  a.sign = sign;
+  std::vector<int> rad;
+  int L = W;
+  if (op.cls != CLS_EMPTY && !factor(W, rad)) {
+    a.bluestein = 1;
+    L = next_pow2(2 * W - 1);
+    factor(L, rad);
+  }
+  a.L = L;
+  a.nrad = (int)rad.size();
+  for (size_t i = 0; i < rad.size(); i++) a.radix[i] = rad[i];
+  // constant tables (long double; rounded to the working precision on upload)
+  ps.tw.clear();
+  ps.chirp.clear();
+  ps.chirp_hat.clear();
+  ps.pre.clear();
+  ps.post.clear();
+  if (op.cls != CLS_EMPTY) {
+    ps.tw.resize(L);
+    for (int t = 0; t < L; t++) ps.tw[t] = expi_pi(-2LL * t, L);
+  }
+  if (a.bluestein) {
+    ps.chirp.resize(W);
+    std::vector<cld> b(L, cld(0, 0));
+    for (int n = 0; n < W; n++) {
+      long long n2 = ((long long)n * n) % (2LL * W);
+      ps.chirp[n] = expi_pi(sign * n2, W);
+    }
+    for (int m = 0; m < W; m++) b[m] = std::conj(ps.chirp[m]);
+    for (int m = 1; m < W; m++) b[L - m] = std::conj(ps.chirp[m]);
+    host_fft_pow2(b);
+    for (int m = 0; m < L; m++) b[m] /= (long double)L;
+    ps.chirp_hat = b;
+  }
+  a.zero_fill = a.bluestein;
+  if (op.cls == CLS_R2R) {
+    const int k = op.r2r_kind;
+    if (k == RODFT00 || k == REDFT01 || k == RODFT01) a.zero_fill = 1;
+    if (k == REDFT01 || k == RODFT01 || k == REDFT11 || k == RODFT11) {
+      ps.pre.resize(W);
+      for (int m = 0; m < W; m++) ps.pre[m] = expi_pi(m, 2LL * N);  // e^{i pi m / 2N}
+    }
+    const cld I(0, 1);
+    std::vector<cld> post(N);
+    bool need = true;
+    switch (k) {
+      case REDFT00: case REDFT01: need = false; break;
+      case RODFT00: a.post_off = 1; for (int q = 0; q < N; q++) post[q] = I; break;
+      case REDFT10: for (int q = 0; q < N; q++) post[q] = expi_pi(-q, 2LL * N); break;
+      case RODFT10: a.post_off = 1; for (int q = 0; q < N; q++) post[q] = I * expi_pi(-(q + 1), 2LL * N); break;
+      case RODFT01: for (int q = 0; q < N; q++) post[q] = -I; break;
+      case REDFT11: for (int q = 0; q < N; q++) post[q] = expi_pi(2 * q + 1, 4LL * N); break;
+      case RODFT11: for (int q = 0; q < N; q++) post[q] = -I * expi_pi(2 * q + 1, 4LL * N); break;
+    }
+    if (need) {
+      a.post_mode = 1;
+      ps.post = post;
+    }
+  }
+  // tile geometry: e is the input's stride-1 axis when d is strided on
+  // input, else the output's; f is the remaining axis
+  long long is[3], os[3];
+  strides_of(lin, is);
+  strides_of(lout, os);
+  const int o1 = (d + 1) % 3, o2 = (d + 2) % 3;
+  int e;
+  if (is[d] != 1) e = (is[o1] <= is[o2]) ? o1 : o2;
+  else e = (os[o1] <= os[o2]) ? o1 : o2;
+  const int f = 3 - d - e;
+  a.is_d = is[d]; a.is_e = is[e]; a.is_f = is[f];
+  a.os_d = os[d]; a.os_e = os[e]; a.os_f = os[f];
+  a.ne = lin.dims[e];
+  a.nf = lin.dims[f];
+  a.load_order = (is[d] == 1 && nin > 1) ? ORD_LINE : ORD_TILE;
+  a.store_order = (os[d] == 1 && nout > 1) ? ORD_LINE : ORD_TILE;
+  const size_t esz = 2 * (size_t)prec;
+  a.lpad = L + ((L >= 16) ? 1 : 0);
+  const size_t per_line = 2 * (size_t)a.lpad * esz;
+  int B = 1;
+  const int Bmax = std::min(32, next_pow2(a.ne));
+  while (B * 2 <= Bmax && per_line * (B * 2) <= LDS_BUDGET) B *= 2;
+  if (per_line * B > LDS_MAX) {
+    set_error("transform length %d (work length %d) exceeds the LDS capacity of one workgroup", N, L);
+    return false;
+  }
+  a.B = B;
+  a.logB = 0;
+  while ((1 << a.logB) < B) a.logB++;
+  a.ntile_e = (a.ne + B - 1) / B;
+  a.div_nin = make_fastdiv((uint32_t)std::max(nin, 1));
+  a.div_nout = make_fastdiv((uint32_t)std::max(nout, 1));
+  a.div_ntile = make_fastdiv((uint32_t)a.ntile_e);
+  ps.lds_bytes = per_line * B;
+  ps.threads = 256;
+  const unsigned long long nbk = (unsigned long long)a.ntile_e * (unsigned long long)a.nf;
+  if (nbk > 0x7fffffffULL) {
+    set_error("grid too large");
+    return false;
+  }
+  ps.nblocks = (unsigned int)nbk;
+  ps.in_bytes = layout_elems(lin) * elem_bytes(prec, lin.dt);
+  ps.out_bytes = layout_elems(lout) * elem_bytes(prec, lout.dt);
+  ps.inplace_ok = (lin.mo[0] == lout.mo[0] && lin.mo[1] == lout.mo[1] && lin.mo[2] == lout.mo[2] &&
+                   lin.dt == lout.dt && nin == nout);
+  ps.deriv_capable = (op.cls != CLS_EMPTY) && lout.dt == 2;
+  // compute_deriv_loc (build/exec.C:228-287): g = 2(n-1) after R2C, else n
+  a.dg = (op.cls == CLS_R2C) ? 2 * (nout - 1) : nout;
+  a.dmid = a.dg / 2;
+  a.scale = 1.0;
+  return true;
+}
+
+// ------------------------------------------------------------------ chain
+static const int PERMS[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+
+// 1 when one tile shape coalesces both sides of the pass, 4 when the
+// transform axis and both stride-1 axes are all different.
+static int pass_cost(int axis, const int moi[3], const int moo[3]) {
+  const int ai = stride1_axis(moi), ao = stride1_axis(moo);
+  if (ai == axis || ao == axis || ai == ao) return 1;
+  return 4;
+}
+
+static bool same_mo(const int a[3], const int b[3]) { return a[0] == b[0] && a[1] == b[1] && a[2] == b[2]; }
+
+bool plan_chain(int prec, const OpSpec ops[3], const int order[3], const Layout &in, const Layout &out,
+                PassChain &chain) {
+  chain.passes.clear();
+  chain.copy_only = false;
+  chain.desc.clear();
+  for (int a = 0; a < 3; a++) chain.pass_of_axis[a] = -1;
+  std::vector<int> axes;
+  for (int i = 0; i < 3; i++)
+    if (ops[order[i]].cls != CLS_EMPTY) axes.push_back(order[i]);
+  const int k = (int)axes.size();
+  if (k == 0) {
+    for (int b = 0; b < 3; b++)
+      if (in.dims[b] != out.dims[b]) {
+        set_error("EMPTY transform: input and output dims differ");
+        return false;
+      }
+    if (in.dt != out.dt) {
+      set_error("EMPTY transform: input and output element types differ");
+      return false;
+    }
+    if (same_mo(in.mo, out.mo)) {
+      chain.copy_only = true;
+      chain.desc = "copy";
+      return true;
+    }
+    PassDesc ps;
+    OpSpec cp = {CLS_EMPTY, -1, 0, in.dt, in.dt};
+    if (!describe_pass(prec, stride1_axis(out.mo), cp, in, out, ps)) return false;
+    chain.passes.push_back(ps);
+    chain.desc = "reorder";
+    return true;
+  }
+  std::vector<Layout> lay(k + 1);
+  lay[0] = in;
+  for (int i = 0; i < k; i++) {
+    Layout l = lay[i];
+    const int ax = axes[i];
+    const OpSpec &op = ops[ax];
+    if (op.cls == CLS_R2C) l.dims[ax] = l.dims[ax] / 2 + 1;
+    else if (op.cls == CLS_C2R) l.dims[ax] = out.dims[ax];
+    if (op.in_dt != l.dt) {
+      set_error("transform along axis %d expects %s input", ax, op.in_dt == 2 ? "complex" : "real");
+      return false;
+    }
+    l.dt = op.out_dt;
+    lay[i + 1] = l;
+  }
+  for (int b = 0; b < 3; b++)
+    if (lay[k].dims[b] != out.dims[b]) {
+      set_error("output grid does not match the transform: axis %d has %d, transform gives %d", b,
+                out.dims[b], lay[k].dims[b]);
+      return false;
+    }
+  if (lay[k].dt != out.dt) {
+    set_error("output element type does not match the transform");
+    return false;
+  }
+  // intermediate memory orders: exhaustive over 6^(k-1); candidates are
+  // tried with the input order first so that ties keep it
+  std::vector<int> cand;
+  for (int q = 0; q < 6; q++)
+    if (same_mo(PERMS[q], in.mo)) cand.insert(cand.begin(), q);
+    else cand.push_back(q);
+  int ncomb = 1;
+  for (int i = 1; i < k; i++) ncomb *= 6;
+  int best_cost = 1 << 30;
+  std::vector<int> best(k + 1, 0);
+  for (int c = 0; c < ncomb; c++) {
+    int cc = c;
+    const int *mo[4];
+    mo[0] = in.mo;
+    mo[k] = out.mo;
+    std::vector<int> sel(k + 1, 0);
+    for (int i = 1; i < k; i++) {
+      sel[i] = cand[cc % 6];
+      cc /= 6;
+      mo[i] = PERMS[sel[i]];
+    }
+    int cost = 0;
+    for (int i = 0; i < k; i++) cost += pass_cost(axes[i], mo[i], mo[i + 1]);
+    if (cost < best_cost) {
+      best_cost = cost;
+      best = sel;
+    }
+  }
+  for (int i = 1; i < k; i++) memcpy(lay[i].mo, PERMS[best[i]], sizeof(lay[i].mo));
+  memcpy(lay[k].mo, out.mo, sizeof(lay[k].mo));
+  char buf[256];
+  for (int i = 0; i < k; i++) {
+    PassDesc ps;
+    if (!describe_pass(prec, axes[i], ops[axes[i]], lay[i], lay[i + 1], ps)) {
+      chain.passes.clear();
+      return false;
+    }
+    chain.pass_of_axis[axes[i]] = (int)chain.passes.size();
+    snprintf(buf, sizeof(buf), "%s%s[axis %d, N=%d%s, mo (%d%d%d)->(%d%d%d), B=%d]", i ? " | " : "",
+             cls_name(ops[axes[i]].cls), axes[i], ps.args.N, ps.args.bluestein ? ", bluestein" : "",
+             lay[i].mo[0], lay[i].mo[1], lay[i].mo[2], lay[i + 1].mo[0], lay[i + 1].mo[1], lay[i + 1].mo[2],
+             ps.args.B);
+    chain.desc += buf;
+    chain.passes.push_back(ps);
+  }
+  return true;
+}
+
+// ------------------------------------------------------------------ buffers
+void assign_buffers(const std::vector<PassDesc> &passes, size_t out_cap, bool alias, int dst[8], size_t need[2],
+                    bool *copy_back) {
+  const int k = (int)passes.size();
+  need[0] = need[1] = 0;
+  *copy_back = false;
+  if (k == 0) return;
+  int ncomb = 1;
+  for (int i = 0; i < k - 1; i++) ncomb *= 3;
+  size_t best_score = (size_t)-1;
+  bool found = false;
+  auto phys = [&](int b) { return (alias && b == BUF_OUT) ? BUF_IN : b; };
+  for (int c = 0; c < ncomb; c++) {
+    int d[8];
+    int cc = c;
+    for (int i = 0; i < k - 1; i++) {
+      d[i] = BUF_OUT + (cc % 3);
+      cc /= 3;
+    }
+    d[k - 1] = BUF_OUT;
+    bool ok = true;
+    size_t nd[2] = {0, 0};
+    int src = BUF_IN;
+    for (int i = 0; i < k && ok; i++) {
+      const PassDesc &ps = passes[i];
+      if (phys(d[i]) == phys(src) && !ps.inplace_ok) ok = false;
+      if (d[i] == BUF_OUT && ps.out_bytes > out_cap) ok = false;
+      if (d[i] == BUF_S0) nd[0] = std::max(nd[0], ps.out_bytes);
+      if (d[i] == BUF_S1) nd[1] = std::max(nd[1], ps.out_bytes);
+      src = d[i];
+    }
+    if (!ok) continue;
+    const size_t score = nd[0] + nd[1];
+    if (score < best_score) {
+      best_score = score;
+      found = true;
+      for (int i = 0; i < k; i++) dst[i] = d[i];
+      need[0] = nd[0];
+      need[1] = nd[1];
+    }
+  }
+  if (!found) {
+    // e.g. a single pass that cannot run in place with in == out
+    for (int i = 0; i < k; i++) dst[i] = (i % 2 == 0) ? BUF_S0 : BUF_S1;
+    need[0] = need[1] = 0;
+    for (int i = 0; i < k; i++) {
+      const int s = dst[i] - BUF_S0;
+      need[s] = std::max(need[s], passes[i].out_bytes);
+    }
+    *copy_back = true;
+  }
+}
+
+}  // namespace p3e

@@ -1,0 +1,75 @@
+// Pass planning for libp3dfft3_hip.so, in plain C++ (no HIP calls): radix
+// plans, constant tables in long double, tile geometry, the memory order of
+// every intermediate array, and the assignment of passes to buffers.  The
+// engine uploads the tables and launches; the CPU tests run the same plans
+// through the host build of the kernel code.
+#pragma once
+#include <complex>
+#include <string>
+#include <vector>
+
+#include "../kernels/pass_args.h"
+
+namespace p3e {
+
+typedef std::complex<long double> cld;
+
+// A 1D operation along one logical axis.
+struct OpSpec {
+  int cls;       // CLS_*
+  int sign;      // C2C: -1 forward, +1 backward
+  int r2r_kind;  // for CLS_R2R
+  int in_dt;     // 1 real, 2 complex
+  int out_dt;
+};
+
+// An array: logical dims, MemOrder (storage rank of each logical axis; 0 is
+// stride-1), and element type (1 real, 2 complex).
+struct Layout {
+  int dims[3];
+  int mo[3];
+  int dt;
+};
+
+struct PassDesc {
+  PassArgs args;  // table pointers and in/out are filled by the executor
+  int axis;
+  Layout lin, lout;
+  size_t in_bytes, out_bytes;
+  size_t lds_bytes;
+  int threads;
+  unsigned int nblocks;
+  bool inplace_ok;
+  bool deriv_capable;  // complex output along a transformed axis
+  std::vector<cld> tw, chirp, chirp_hat, pre, post;
+};
+
+struct PassChain {
+  std::vector<PassDesc> passes;
+  int pass_of_axis[3];
+  bool copy_only;  // no transform and identical layouts
+  std::string desc;
+};
+
+size_t layout_elems(const Layout &l);
+size_t elem_bytes(int prec, int dt);
+
+bool describe_pass(int prec, int axis, const OpSpec &op, const Layout &lin, const Layout &lout, PassDesc &ps);
+// ops[a]: operation on logical axis a; order: axes in execution order.
+bool plan_chain(int prec, const OpSpec ops[3], const int order[3], const Layout &in, const Layout &out,
+                PassChain &chain);
+
+enum { BUF_IN = 0, BUF_OUT = 1, BUF_S0 = 2, BUF_S1 = 3 };
+// Destination buffer of each pass (BUF_OUT or scratch), and scratch sizes.
+// alias: caller's in and out are the same array.  copy_back: the last pass
+// writes scratch and the result must be copied to out.
+void assign_buffers(const std::vector<PassDesc> &passes, size_t out_cap, bool alias, int dst[8],
+                    size_t need[2], bool *copy_back);
+
+// errors (shared with the engine)
+void set_error(const char *fmt, ...);
+const char *last_error();
+int last_error_code();
+void clear_error();
+
+}  // namespace p3e

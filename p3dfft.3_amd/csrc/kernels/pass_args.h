@@ -1,0 +1,72 @@
+// Launch arguments of the fused reorder + 1D transform pass kernel.
+// Shared by the host planner (C++) and the HIP kernels; plain data only.
+#pragma once
+#include <stdint.h>
+
+namespace p3e {
+
+enum Cls { CLS_EMPTY = 0, CLS_R2C = 1, CLS_C2R = 2, CLS_C2C = 3, CLS_R2R = 4 };
+
+// FFTW r2r kinds (same numbering as the oracle)
+enum R2RKind {
+  REDFT00 = 0, REDFT10 = 1, REDFT01 = 2, REDFT11 = 3,
+  RODFT00 = 4, RODFT10 = 5, RODFT01 = 6, RODFT11 = 7
+};
+
+// Unsigned division by a runtime constant: q = (umulhi(n, mul) + n) >> shift,
+// exact for 0 <= n < 2^31 (Granlund-Montgomery round-up method).
+struct FastDiv {
+  uint32_t d, mul, shift;
+};
+
+static inline FastDiv make_fastdiv(uint32_t d) {
+  FastDiv f;
+  f.d = d;
+  uint32_t s = 0;
+  while ((1ull << s) < d) s++;
+  f.shift = s;
+  f.mul = (uint32_t)(((1ull << 32) * ((1ull << s) - d)) / d + 1);
+  return f;
+}
+
+#define P3E_MAX_RADIX_PASSES 24
+
+// Load/store orderings of the tile, chosen from the strides so that
+// consecutive threads touch consecutive addresses.
+enum Order { ORD_LINE = 0, ORD_TILE = 1 };
+
+struct PassArgs {
+  const void *in;
+  void *out;
+  const void *tw;         // L entries: exp(-2 pi i t / L)
+  const void *chirp;      // W entries (Bluestein): exp(sign i pi n^2 / W)
+  const void *chirp_hat;  // L entries: DFT_L of the conjugate chirp, scaled 1/L
+  const void *pre;        // r2r pre-twiddle, W entries (or null)
+  const void *post;       // r2r post-twiddle, nout entries (or null)
+  long long is_d, is_e, is_f;  // input element strides (units of input elements)
+  long long os_d, os_e, os_f;  // output element strides
+  int cls, r2r_kind;
+  int in_cplx, out_cplx;
+  int N;           // transform length (real length for R2C/C2R)
+  int nin, nout;   // elements along the transform axis, input / output
+  int W;           // inner complex DFT length
+  int L;           // length of the FFT done in LDS (W, or the Bluestein length)
+  int sign;        // sign of the inner DFT of length W
+  int bluestein;
+  int nrad;
+  int radix[P3E_MAX_RADIX_PASSES];
+  int ne, nf;      // extents of the tile axis e and the outer axis f
+  int B, logB;     // lines per workgroup (power of two)
+  int ntile_e;
+  int lpad;        // LDS pitch of one line, in complex elements
+  int zero_fill;   // clear the line before scattering the extended input
+  int post_off;    // output k reads work element k + post_off
+  int post_mode;   // 0: none, 1: multiply by post[k]
+  int load_order, store_order;
+  int deriv, dg, dmid;  // spectral derivative (compute_deriv_loc semantics)
+  int has_scale;
+  double scale;
+  FastDiv div_nin, div_nout, div_ntile;
+};
+
+}  // namespace p3e

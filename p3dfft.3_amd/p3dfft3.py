@@ -1,0 +1,207 @@
+"""ctypes binding of libp3dfft3_hip.so's C ABI (include/Cwrap.h and
+include/p3dfft_hip.h), for Python callers such as the tests and bench.py.
+
+The names follow the reference's C interface (p3dfft_setup,
+p3dfft_init_proc_grid, p3dfft_init_data_grid, p3dfft_plan_3Dtrans,
+p3dfft_exec_3Dtrans_double, ...).  Array arguments are raw addresses: device
+pointers (e.g. torch.Tensor.data_ptr() of a cuda tensor) are used in place,
+host pointers are staged by the library.  There is no CPU fallback: if the
+HIP library is missing this module raises on load.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libp3dfft3_hip.so")
+
+MPI_COMM_WORLD = 0x44000000  # MPICH's handle; the library never calls MPI with it
+
+TYPE_NAMES = [
+    "EMPTY_TYPE_SINGLE", "EMPTY_TYPE_DOUBLE", "EMPTY_TYPE_SINGLE_COMPLEX", "EMPTY_TYPE_DOUBLE_COMPLEX",
+    "R2CFFT_S", "R2CFFT_D", "C2RFFT_S", "C2RFFT_D",
+    "CFFT_FORWARD_S", "CFFT_FORWARD_D", "CFFT_BACKWARD_S", "CFFT_BACKWARD_D",
+] + [f"{k}{i}_{c}_{p}" for i in (1, 2, 3, 4) for k in ("DCT", "DST") for c in ("REAL", "COMPLEX")
+     for p in ("S", "D")]
+
+
+class Grid(ctypes.Structure):
+    """struct CDataGrid_struct (include/Cwrap.h)."""
+    _fields_ = [("nd", ctypes.c_int), ("Gdims", ctypes.c_int * 3), ("dim_conj_sym", ctypes.c_int),
+                ("MemOrder", ctypes.c_int * 3), ("Ldims", ctypes.c_int * 3), ("Dmap", ctypes.c_int * 3),
+                ("pgrid", ctypes.c_int), ("grid_id", ctypes.c_int * 3), ("GlobStart", ctypes.c_int * 3),
+                ("taskid", ctypes.c_int), ("numtasks", ctypes.c_int), ("ProcDims", ctypes.c_int * 3),
+                ("mpi_comm_glob", ctypes.c_int)]
+
+
+GridP = ctypes.POINTER(Grid)
+I3 = ctypes.c_int * 3
+VP = ctypes.c_void_p
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        L.p3dfft_init_data_grid.restype = GridP
+        L.p3dfft_init_data_grid.argtypes = [I3, ctypes.c_int, ctypes.c_int, I3, I3]
+        L.p3dfft_init_proc_grid.argtypes = [I3, ctypes.c_int]
+        L.p3dfft_plan_3Dtrans.argtypes = [GridP, GridP, ctypes.c_int]
+        L.p3dfft_plan_1Dtrans.argtypes = [GridP, GridP, ctypes.c_int, ctypes.c_int]
+        L.p3dfft_free_data_grid.argtypes = [GridP]
+        for n in ("double", "single"):
+            getattr(L, f"p3dfft_exec_3Dtrans_{n}").argtypes = [ctypes.c_int, VP, VP, ctypes.c_int]
+            getattr(L, f"p3dfft_exec_3Dderiv_{n}").argtypes = [ctypes.c_int, VP, VP, ctypes.c_int, ctypes.c_int]
+            getattr(L, f"p3dfft_exec_1Dtrans_{n}").argtypes = [ctypes.c_int, VP, VP, ctypes.c_int]
+            getattr(L, f"p3dfft_compute_deriv_{n}").argtypes = [VP, VP, GridP, ctypes.c_int]
+            getattr(L, f"p3dfft_exec_3Dtrans_{n}_async").argtypes = [ctypes.c_int, VP, VP, ctypes.c_int,
+                                                                     ctypes.c_int, VP]
+        L.p3dfft_hip_last_error.argtypes = [ctypes.c_char_p, ctypes.c_int]
+        L.p3dfft_hip_plan_describe.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+        L.p3dfft_hip_pass_bytes.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong),
+                                            ctypes.POINTER(ctypes.c_longlong)]
+        L.p3dfft_hip_stream.restype = VP
+        _lib = L
+    return _lib
+
+
+def setup():
+    lib().p3dfft_setup()
+
+
+def cleanup():
+    lib().p3dfft_cleanup()
+
+
+def type_id(name):
+    """Value of the P3DFFT_<name> global after setup()."""
+    return ctypes.c_int.in_dll(lib(), "P3DFFT_" + name).value
+
+
+def last_error(clear=True):
+    buf = ctypes.create_string_buffer(1024)
+    rc = lib().p3dfft_hip_last_error(buf, 1024)
+    if clear:
+        lib().p3dfft_hip_clear_error()
+    return rc, buf.value.decode()
+
+
+def check():
+    rc, msg = last_error()
+    if rc:
+        raise RuntimeError(msg)
+
+
+def init_3Dtype(types):
+    return lib().p3dfft_init_3Dtype(I3(*types))
+
+
+def init_proc_grid(pdims=(1, 1, 1), comm=MPI_COMM_WORLD):
+    return lib().p3dfft_init_proc_grid(I3(*pdims), comm)
+
+
+def init_data_grid(gdims, dim_conj_sym, pgrid, dmap, mem_order):
+    g = lib().p3dfft_init_data_grid(I3(*gdims), dim_conj_sym, pgrid, I3(*dmap), I3(*mem_order))
+    if not g:
+        check()
+        raise RuntimeError("p3dfft_init_data_grid failed")
+    return g
+
+
+def free_data_grid(g):
+    lib().p3dfft_free_data_grid(g)
+
+
+def free_proc_grid(p):
+    lib().p3dfft_free_proc_grid(p)
+
+
+def plan_3Dtrans(g1, g2, type3d):
+    h = lib().p3dfft_plan_3Dtrans(g1, g2, type3d)
+    check()
+    if h < 0:
+        raise RuntimeError("p3dfft_plan_3Dtrans failed")
+    return h
+
+
+def plan_1Dtrans(g1, g2, type_id_, d):
+    h = lib().p3dfft_plan_1Dtrans(g1, g2, type_id_, d)
+    check()
+    if h < 0:
+        raise RuntimeError("p3dfft_plan_1Dtrans failed")
+    return h
+
+
+def exec_3Dtrans(plan, inp, out, OW=0, prec=8):
+    f = lib().p3dfft_exec_3Dtrans_double if prec == 8 else lib().p3dfft_exec_3Dtrans_single
+    f(plan, inp, out, OW)
+    check()
+
+
+def exec_3Dderiv(plan, inp, out, idir, OW=0, prec=8):
+    f = lib().p3dfft_exec_3Dderiv_double if prec == 8 else lib().p3dfft_exec_3Dderiv_single
+    f(plan, inp, out, idir, OW)
+    check()
+
+
+def exec_1Dtrans(plan, inp, out, OW=0, prec=8):
+    f = lib().p3dfft_exec_1Dtrans_double if prec == 8 else lib().p3dfft_exec_1Dtrans_single
+    f(plan, inp, out, OW)
+    check()
+
+
+def compute_deriv(inp, out, grid, idir, prec=8):
+    f = lib().p3dfft_compute_deriv_double if prec == 8 else lib().p3dfft_compute_deriv_single
+    f(inp, out, grid, idir)
+    check()
+
+
+def exec_async(plan, inp, out, idir=-1, OW=0, stream=None, prec=8):
+    f = lib().p3dfft_exec_3Dtrans_double_async if prec == 8 else lib().p3dfft_exec_3Dtrans_single_async
+    rc = f(plan, inp, out, idir, OW, stream)
+    if rc:
+        check()
+        raise RuntimeError("async exec failed")
+
+
+def describe(plan):
+    buf = ctypes.create_string_buffer(2048)
+    lib().p3dfft_hip_plan_describe(plan, buf, 2048)
+    return buf.value.decode()
+
+
+def npasses(plan):
+    return lib().p3dfft_hip_plan_npasses(plan)
+
+
+def pass_bytes(plan, i):
+    a, b = ctypes.c_longlong(), ctypes.c_longlong()
+    rc = lib().p3dfft_hip_pass_bytes(plan, i, ctypes.byref(a), ctypes.byref(b))
+    if rc:
+        raise IndexError(i)
+    return a.value, b.value
+
+
+def stream():
+    return lib().p3dfft_hip_stream()
+
+
+def profile(plan, enable=True):
+    """Start (and reset) per-pass HIP-event timing of a 3D plan."""
+    L = lib()
+    L.p3dfft_hip_profile.argtypes = [ctypes.c_int, ctypes.c_int]
+    return L.p3dfft_hip_profile(plan, int(enable))
+
+
+def profile_read(plan, npass=8):
+    """(number of recorded executions, [avg ms per pass], total ms)."""
+    L = lib()
+    L.p3dfft_hip_profile_read.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+                                          ctypes.POINTER(ctypes.c_double)]
+    avg = (ctypes.c_double * npass)()
+    tot = ctypes.c_double()
+    n = L.p3dfft_hip_profile_read(plan, avg, npass, ctypes.byref(tot))
+    return n, list(avg)[:max(0, npasses(plan))], tot.value

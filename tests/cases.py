@@ -1,0 +1,49 @@
+"""Shared case construction for the parity tests (CPU host-build and GPU)."""
+import numpy as np
+
+import oracle_binding as orc
+
+PERMS = [[0, 1, 2], [0, 2, 1], [1, 0, 2], [1, 2, 0], [2, 0, 1], [2, 1, 0]]
+
+# type IDs (registration order of the reference's setup())
+EMPTY_S, EMPTY_D, EMPTY_SC, EMPTY_DC = 0, 1, 2, 3
+R2C_S, R2C_D, C2R_S, C2R_D = 4, 5, 6, 7
+CF_S, CF_D, CB_S, CB_D = 8, 9, 10, 11
+
+
+def r2r_id(kind_name, complex_, prec):
+    """kind_name in DCT1..DCT4, DST1..DST4 -> type ID."""
+    base = {"DCT1": 12, "DST1": 16, "DCT2": 20, "DST2": 24, "DCT3": 28, "DST3": 32,
+            "DCT4": 36, "DST4": 40}[kind_name]
+    return base + (2 if complex_ else 0) + (1 if prec == 8 else 0)
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.complex128).ravel()
+    b = np.asarray(b, dtype=np.complex128).ravel()
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
+
+
+def out_dims(types, dims, c2r_len=None):
+    """Logical output dims of a 3D transform: R2C axis -> n/2+1; C2R axis ->
+    c2r_len (the real length) or 2(n-1)."""
+    o = list(dims)
+    for a, t in enumerate(types):
+        ti = orc.type_info(t)
+        if ti.dt1 < ti.dt2:
+            o[a] = dims[a] // 2 + 1
+        elif ti.dt1 > ti.dt2:
+            o[a] = c2r_len if c2r_len is not None else 2 * (dims[a] - 1)
+    return o
+
+
+def random_input(types, dims, mo, seed, hermitian_axis=None):
+    """Storage-order input for a 3D transform with the given types."""
+    rng = np.random.default_rng(seed)
+    order = orc.find_order(types, mo, mo)
+    t0 = orc.type_info(types[order[0]])
+    shape = orc.storage_shape(dims, mo)
+    if t0.dt1 == 1:
+        return rng.uniform(-1.0, 1.0, shape)
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)

@@ -1,0 +1,128 @@
+// Host build of the pass kernel, for CPU tests only (never part of the
+// product library).  It plans with the product's pass_plan.cpp and runs each
+// workgroup of each pass through the same run_tile() the HIP kernel runs,
+// one workgroup at a time with a single thread and a no-op barrier.  This
+// checks the kernel's indexing, radix passes, extension rules and layout
+// changes without a GPU.
+#include <string.h>
+
+#include <vector>
+
+#include "../../p3dfft.3_amd/csrc/host/pass_plan.h"
+#include "../../p3dfft.3_amd/csrc/kernels/fft_core.h"
+
+using namespace p3e;
+
+struct HostBarrier {
+  void operator()() const {}
+};
+
+template <typename T> static std::vector<T> tab(const std::vector<cld> &v) {
+  std::vector<T> h(2 * v.size());
+  for (size_t i = 0; i < v.size(); i++) {
+    h[2 * i] = (T)v[i].real();
+    h[2 * i + 1] = (T)v[i].imag();
+  }
+  return h;
+}
+
+template <typename T> static int run_pass(const PassDesc &pd, const void *in, void *out, int deriv) {
+  std::vector<T> tw = tab<T>(pd.tw), ch = tab<T>(pd.chirp), chh = tab<T>(pd.chirp_hat), pre = tab<T>(pd.pre),
+                 post = tab<T>(pd.post);
+  PassArgs a = pd.args;
+  a.in = in;
+  a.out = out;
+  a.tw = tw.empty() ? nullptr : tw.data();
+  a.chirp = ch.empty() ? nullptr : ch.data();
+  a.chirp_hat = chh.empty() ? nullptr : chh.data();
+  a.pre = pre.empty() ? nullptr : pre.data();
+  a.post = post.empty() ? nullptr : post.data();
+  a.deriv = deriv;
+  std::vector<cx<T>> A((size_t)a.B * a.lpad), B((size_t)a.B * a.lpad);
+  for (unsigned int t = 0; t < pd.nblocks; t++) {
+    // poison LDS between tiles so stale data cannot hide a missing write
+    for (auto &v : A) v = cx<T>{(T)1e30, (T)-1e30};
+    for (auto &v : B) v = cx<T>{(T)-1e30, (T)1e30};
+    run_tile<T>(a, t, A.data(), B.data(), 0, 1, HostBarrier());
+  }
+  return 0;
+}
+
+extern "C" {
+
+// Plans and runs a transform on host arrays.  ops_*[a]: class, sign, r2r
+// kind, input/output element type of the operation on logical axis a.
+// Returns 0, or -1 with the planner's message in emu_error().
+int emu_exec(int prec, const int *cls, const int *sign, const int *kind, const int *idt, const int *odt,
+             const int *order, const int *in_dims, const int *in_mo, int in_dt, const int *out_dims,
+             const int *out_mo, int out_dt, const void *in, void *out, int idir, char *desc, int desc_len) {
+  OpSpec ops[3];
+  for (int a = 0; a < 3; a++) ops[a] = OpSpec{cls[a], sign[a], kind[a], idt[a], odt[a]};
+  Layout li, lo;
+  for (int i = 0; i < 3; i++) {
+    li.dims[i] = in_dims[i];
+    li.mo[i] = in_mo[i];
+    lo.dims[i] = out_dims[i];
+    lo.mo[i] = out_mo[i];
+  }
+  li.dt = in_dt;
+  lo.dt = out_dt;
+  PassChain ch;
+  if (!plan_chain(prec, ops, order, li, lo, ch)) return -1;
+  if (desc && desc_len > 0) {
+    strncpy(desc, ch.desc.c_str(), desc_len - 1);
+    desc[desc_len - 1] = 0;
+  }
+  const size_t out_bytes = layout_elems(lo) * elem_bytes(prec, lo.dt);
+  if (ch.copy_only) {
+    memcpy(out, in, out_bytes);
+    return 0;
+  }
+  std::vector<std::vector<char>> bufs(ch.passes.size());
+  const void *src = in;
+  for (size_t i = 0; i < ch.passes.size(); i++) {
+    const PassDesc &pd = ch.passes[i];
+    void *dst;
+    if (i + 1 == ch.passes.size()) dst = out;
+    else {
+      bufs[i].assign(pd.out_bytes, 0);
+      dst = bufs[i].data();
+    }
+    const int deriv = (idir >= 0 && idir == pd.axis && pd.deriv_capable) ? 1 : 0;
+    if (prec == 8) run_pass<double>(pd, src, dst, deriv);
+    else run_pass<float>(pd, src, dst, deriv);
+    src = dst;
+  }
+  return 0;
+}
+
+const char *emu_error(void) { return last_error(); }
+
+// Buffer assignment of a plan, for tests: dst[i] per pass, scratch sizes.
+int emu_assign(int prec, const int *cls, const int *sign, const int *kind, const int *idt, const int *odt,
+               const int *order, const int *in_dims, const int *in_mo, int in_dt, const int *out_dims,
+               const int *out_mo, int out_dt, long long out_cap, int alias, int *dst, long long *need,
+               int *copy_back) {
+  OpSpec ops[3];
+  for (int a = 0; a < 3; a++) ops[a] = OpSpec{cls[a], sign[a], kind[a], idt[a], odt[a]};
+  Layout li, lo;
+  for (int i = 0; i < 3; i++) {
+    li.dims[i] = in_dims[i];
+    li.mo[i] = in_mo[i];
+    lo.dims[i] = out_dims[i];
+    lo.mo[i] = out_mo[i];
+  }
+  li.dt = in_dt;
+  lo.dt = out_dt;
+  PassChain ch;
+  if (!plan_chain(prec, ops, order, li, lo, ch)) return -1;
+  size_t nd[2];
+  bool cb;
+  assign_buffers(ch.passes, (size_t)out_cap, alias != 0, dst, nd, &cb);
+  need[0] = (long long)nd[0];
+  need[1] = (long long)nd[1];
+  *copy_back = cb;
+  return (int)ch.passes.size();
+}
+
+}  // extern "C"

@@ -1,0 +1,151 @@
+"""The product's planner and kernel code, built for the host and run one
+workgroup at a time, against the oracle.  Catches indexing, radix, layout
+and extension-rule mistakes on the CPU; the GPU tests then check the same
+plans on the device."""
+import numpy as np
+import pytest
+
+import cases as C
+import emu_binding as emu
+import oracle_binding as orc
+
+TOL = {8: 1e-12, 4: 1e-5}
+
+
+def run_both_3d(types, dims, mo1, mo2, prec, idir=-1, seed=0, c2r_len=None):
+    od = C.out_dims(types, dims, c2r_len)
+    x = C.random_input(types, dims, mo1, seed)
+    if prec == 4:
+        x = x.astype(np.complex64 if np.iscomplexobj(x) else np.float32)
+    ref = orc.exec3d(types, dims, mo1, mo2, x.astype(np.complex128 if np.iscomplexobj(x) else np.float64),
+                     idir=idir)
+    got, desc = emu.exec3d(types, dims, mo1, od, mo2, x, prec, idir=idir)
+    return got, ref, desc
+
+
+@pytest.mark.parametrize("mo1", C.PERMS)
+@pytest.mark.parametrize("mo2", C.PERMS)
+def test_r2c_3d_all_memory_orders(mo1, mo2):
+    got, ref, desc = run_both_3d([C.R2C_D, C.CF_D, C.CF_D], [12, 10, 6], mo1, mo2, 8)
+    assert C.rel_l2(got, ref) < TOL[8], desc
+
+
+@pytest.mark.parametrize("mo1", C.PERMS)
+def test_c2r_3d_memory_orders(mo1):
+    # backward from a spectrum; C2R length 12 on axis 0
+    types = [C.C2R_D, C.CB_D, C.CB_D]
+    dims = [7, 10, 6]
+    for mo2 in ([0, 1, 2], [2, 1, 0], [1, 2, 0]):
+        rng = np.random.default_rng(3)
+        x = rng.standard_normal(orc.storage_shape(dims, mo1)) + 1j * rng.standard_normal(
+            orc.storage_shape(dims, mo1))
+        ref = orc.exec3d(types, dims, mo1, mo2, x)
+        got, desc = emu.exec3d(types, dims, mo1, [12, 10, 6], mo2, x, 8)
+        assert C.rel_l2(got, ref) < TOL[8], (mo1, mo2, desc)
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+@pytest.mark.parametrize("dims", [[16, 8, 4], [9, 7, 5], [32, 6, 3], [17, 11, 13], [64, 2, 1]])
+def test_c2c_3d_sizes(prec, dims):
+    t = [C.CF_D] * 3 if prec == 8 else [C.CF_S] * 3
+    got, ref, desc = run_both_3d(t, dims, [0, 1, 2], [2, 1, 0], prec)
+    assert C.rel_l2(got, ref) < TOL[prec], desc
+
+
+@pytest.mark.parametrize("n", [2, 3, 4, 5, 6, 7, 8, 9, 11, 12, 13, 15, 16, 17, 25, 31, 32, 37, 48, 60, 64,
+                               96, 97, 120, 128, 139, 199, 256, 343, 385, 512])
+def test_c2c_1d_lengths(n):
+    dims = [n, 3, 2]
+    mo = [0, 1, 2]
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(orc.storage_shape(dims, mo)) + 1j * rng.standard_normal(orc.storage_shape(dims, mo))
+    for tid in (C.CF_D, C.CB_D):
+        ref = orc.exec_1d(tid, 0, dims, mo, dims, mo, x)
+        got = emu.exec1d(tid, 0, dims, mo, dims, mo, x, 8)
+        assert C.rel_l2(got, ref) < TOL[8], (n, tid)
+
+
+@pytest.mark.parametrize("n", [2, 3, 5, 8, 9, 16, 17, 31, 64, 100, 139, 512])
+def test_r2c_c2r_1d(n):
+    dims = [n, 4, 3]
+    h = n // 2 + 1
+    rng = np.random.default_rng(n)
+    for mo1, mo2 in (([0, 1, 2], [0, 1, 2]), ([1, 0, 2], [0, 2, 1]), ([0, 1, 2], [2, 1, 0])):
+        x = rng.uniform(-1, 1, orc.storage_shape(dims, mo1))
+        ref = orc.exec_1d(C.R2C_D, 0, dims, mo1, [h, 4, 3], mo2, x)
+        got = emu.exec1d(C.R2C_D, 0, dims, mo1, [h, 4, 3], mo2, x, 8)
+        assert C.rel_l2(got, ref) < TOL[8]
+        z = rng.standard_normal(orc.storage_shape([h, 4, 3], mo2)) + 1j * rng.standard_normal(
+            orc.storage_shape([h, 4, 3], mo2))
+        ref = orc.exec_1d(C.C2R_D, 0, [h, 4, 3], mo2, dims, mo1, z)
+        got = emu.exec1d(C.C2R_D, 0, [h, 4, 3], mo2, dims, mo1, z, 8)
+        assert C.rel_l2(got, ref) < TOL[8]
+
+
+KINDS = ["DCT1", "DST1", "DCT2", "DST2", "DCT3", "DST3", "DCT4", "DST4"]
+
+
+@pytest.mark.parametrize("kind", KINDS)
+@pytest.mark.parametrize("n", [2, 3, 5, 8, 16, 17, 33, 65, 100, 129, 513])
+@pytest.mark.parametrize("cplx", [False, True])
+def test_r2r_1d(kind, n, cplx):
+    if kind in ("DCT1", "DCT4") and n < 2:
+        pytest.skip("DCT-I needs n >= 2")
+    tid = C.r2r_id(kind, cplx, 8)
+    dims = [n, 3, 2]
+    rng = np.random.default_rng(n + len(kind))
+    for mo1, mo2 in (([0, 1, 2], [0, 1, 2]), ([2, 0, 1], [0, 2, 1])):
+        shp = orc.storage_shape(dims, mo1)
+        x = rng.uniform(-1, 1, shp) if not cplx else rng.standard_normal(shp) + 1j * rng.standard_normal(shp)
+        ref = orc.exec_1d(tid, 0, dims, mo1, dims, mo2, x)
+        got = emu.exec1d(tid, 0, dims, mo1, dims, mo2, x, 8)
+        assert C.rel_l2(got, ref) < TOL[8], (kind, n, cplx, mo1, mo2)
+
+
+@pytest.mark.parametrize("idir", [0, 1, 2])
+@pytest.mark.parametrize("mo2", [[1, 2, 0], [0, 1, 2], [2, 1, 0]])
+def test_derivative_3d(idir, mo2):
+    got, ref, desc = run_both_3d([C.R2C_D, C.CF_D, C.CF_D], [12, 8, 6], [0, 1, 2], mo2, 8, idir=idir)
+    assert C.rel_l2(got, ref) < TOL[8], desc
+
+
+def test_derivative_odd_c2c():
+    # odd C2C length: mid = (n-1)/2 is zeroed, as compute_deriv_loc does
+    got, ref, desc = run_both_3d([C.CF_D] * 3, [9, 5, 7], [0, 1, 2], [0, 1, 2], 8, idir=2)
+    assert C.rel_l2(got, ref) < TOL[8], desc
+
+
+def test_mixed_r2c_c2c_dct1_complex():
+    # the mixed-type configuration at reduced size: R2C x, C2C y, DCT-I z
+    types = [C.R2C_D, C.CF_D, C.r2r_id("DCT1", True, 8)]
+    for mo1, mo2 in (([0, 1, 2], [1, 2, 0]), ([2, 1, 0], [2, 1, 0])):
+        got, ref, desc = run_both_3d(types, [24, 20, 17], mo1, mo2, 8)
+        assert C.rel_l2(got, ref) < TOL[8], desc
+
+
+def test_empty_axis_2d_transform():
+    # test2D+empty.c: R2C x, EMPTY y, C2C z
+    types = [C.R2C_D, C.EMPTY_DC, C.CF_D]
+    got, ref, desc = run_both_3d(types, [10, 6, 8], [0, 1, 2], [1, 2, 0], 8)
+    assert C.rel_l2(got, ref) < TOL[8], desc
+
+
+def test_all_empty_reorder():
+    types = [C.EMPTY_D] * 3
+    dims = [5, 4, 3]
+    x = np.random.default_rng(1).uniform(-1, 1, orc.storage_shape(dims, [0, 1, 2]))
+    got, desc = emu.exec3d(types, dims, [0, 1, 2], dims, [2, 0, 1], x, 8, order=[0, 1, 2])
+    ref = orc.to_storage(orc.from_storage(x, [0, 1, 2]), [2, 0, 1])
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+def test_r2c_c2r_roundtrip_fp32_fp64(prec):
+    tf = [C.R2C_D, C.CF_D, C.CF_D] if prec == 8 else [C.R2C_S, C.CF_S, C.CF_S]
+    tb = [C.C2R_D, C.CB_D, C.CB_D] if prec == 8 else [C.C2R_S, C.CB_S, C.CB_S]
+    dims = [32, 16, 8]
+    mo1, mo2 = [2, 1, 0], [1, 2, 0]
+    x = np.random.default_rng(7).uniform(-1, 1, orc.storage_shape(dims, mo1))
+    spec, _ = emu.exec3d(tf, dims, mo1, [17, 16, 8], mo2, x, prec)
+    back, _ = emu.exec3d(tb, [17, 16, 8], mo2, dims, mo1, spec, prec)
+    assert C.rel_l2(back / np.prod(dims), x) < (1e-13 if prec == 8 else 1e-6)
