@@ -43,6 +43,11 @@ int p3dfft_hip_pass_bytes(int plan, int i, long long *bytes_in, long long *bytes
 int p3dfft_hip_profile(int plan, int enable);
 int p3dfft_hip_profile_read(int plan, double *avg_ms, int cap, double *total_ms);
 
+/* Plans made after this call use (1, default) or do not use (0) the
+ * power-of-two pass kernel where it applies; 0 forces the generic kernel.
+ * For testing both kernels on the same transforms. */
+void p3dfft_hip_set_fast_path(int enable);
+
 #ifdef __cplusplus
 }
 #endif

@@ -68,9 +68,9 @@ template <typename T> static void *upload(const std::vector<cld> &v, std::vector
 
 static bool upload_tables(int prec, Pass &p) {
   PassArgs &a = p.d.args;
-  const std::vector<cld> *src[5] = {&p.d.tw, &p.d.chirp, &p.d.chirp_hat, &p.d.pre, &p.d.post};
-  const void **dst[5] = {&a.tw, &a.chirp, &a.chirp_hat, &a.pre, &a.post};
-  for (int i = 0; i < 5; i++) {
+  const std::vector<cld> *src[6] = {&p.d.tw, &p.d.chirp, &p.d.chirp_hat, &p.d.pre, &p.d.post, &p.d.wN};
+  const void **dst[6] = {&a.tw, &a.chirp, &a.chirp_hat, &a.pre, &a.post, &a.wN};
+  for (int i = 0; i < 6; i++) {
     if (src[i]->empty()) {
       *dst[i] = nullptr;
       continue;

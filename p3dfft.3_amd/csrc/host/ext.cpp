@@ -71,6 +71,8 @@ int p3dfft_hip_pass_bytes(int plan, int i, long long *bytes_in, long long *bytes
   return 0;
 }
 
+void p3dfft_hip_set_fast_path(int enable) { p3e::set_fast_path(enable != 0); }
+
 int p3dfft_hip_profile(int plan, int enable) {
   p3e::Plan *p = engine_of(plan);
   return p ? p3e::plan_profile(p, enable != 0) : -1;

@@ -134,6 +134,60 @@ static const char *cls_name(int c) {
 static const size_t LDS_BUDGET = 80 * 1024;
 static const size_t LDS_MAX = 160 * 1024;
 
+static bool g_fast_enabled = true;
+void set_fast_path(bool enable) { g_fast_enabled = enable; }
+
+static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+// Switch an already described pass to the power-of-two path when it is a
+// C2C, R2C or C2R of power-of-two complex length M in [16, 4096] (M = N/2 for
+// the real transforms).  Tile: B lines, B as large as the LDS budget and
+// 1024 threads allow (up to 64), P3E_FAST_E elements per thread.
+static void setup_fast(int prec, const OpSpec &op, PassDesc &ps) {
+  PassArgs &a = ps.args;
+  if (!g_fast_enabled || a.bluestein) return;
+  int M;
+  if (op.cls == CLS_C2C) M = a.N;
+  else if ((op.cls == CLS_R2C || op.cls == CLS_C2R) && a.N % 2 == 0) M = a.N / 2;
+  else return;
+  if (!is_pow2(M) || M < 16 || M > 4096) return;
+  std::vector<int> rad;
+  factor(M, rad);
+  for (int r : rad)
+    if (P3E_FAST_E % r) return;
+  const size_t esz = 2 * (size_t)prec;
+  const int lp = M + M / 16;
+  int B = 1;
+  const int Bmax = std::min(64, next_pow2(a.ne));
+  auto lds = [&](int b) { return ((size_t)b * lp + b) * esz; };
+  while (B * 2 <= Bmax && lds(B * 2) <= LDS_BUDGET && (long)M * B * 2 / P3E_FAST_E <= 1024) B *= 2;
+  if (lds(B) > LDS_MAX || (long)M * B / P3E_FAST_E > 1024) return;
+  a.fast = 1;
+  a.M = M;
+  a.logM = 0;
+  while ((1 << a.logM) < M) a.logM++;
+  a.lp = lp;
+  a.nrad = (int)rad.size();
+  for (size_t i = 0; i < rad.size(); i++) a.radix[i] = rad[i];
+  a.L = M;
+  a.zero_fill = 0;
+  a.B = B;
+  a.logB = 0;
+  while ((1 << a.logB) < B) a.logB++;
+  a.ntile_e = (a.ne + B - 1) / B;
+  a.div_ntile = make_fastdiv((uint32_t)a.ntile_e);
+  ps.lds_bytes = lds(B);
+  ps.threads = M * B / P3E_FAST_E;
+  ps.nblocks = (unsigned int)((unsigned long long)a.ntile_e * (unsigned long long)a.nf);
+  ps.tw.resize(M);
+  for (int t = 0; t < M; t++) ps.tw[t] = expi_pi(-2LL * t, M);
+  ps.wN.clear();
+  if (op.cls != CLS_C2C) {
+    ps.wN.resize(M + 1);
+    for (int k = 0; k <= M; k++) ps.wN[k] = expi_pi(-2LL * k, a.N);
+  }
+}
+
 // ------------------------------------------------------------------ one pass
 bool describe_pass(int prec, int axis, const OpSpec &op, const Layout &lin, const Layout &lout, PassDesc &ps) {
   PassArgs &a = ps.args;
@@ -315,6 +369,7 @@ bool describe_pass(int prec, int axis, const OpSpec &op, const Layout &lin, cons
   a.dg = (op.cls == CLS_R2C) ? 2 * (nout - 1) : nout;
   a.dmid = a.dg / 2;
   a.scale = 1.0;
+  setup_fast(prec, op, ps);
   return true;
 }
 
@@ -427,7 +482,8 @@ bool plan_chain(int prec, const OpSpec ops[3], const int order[3], const Layout 
     }
     chain.pass_of_axis[axes[i]] = (int)chain.passes.size();
     snprintf(buf, sizeof(buf), "%s%s[axis %d, N=%d%s, mo (%d%d%d)->(%d%d%d), B=%d]", i ? " | " : "",
-             cls_name(ops[axes[i]].cls), axes[i], ps.args.N, ps.args.bluestein ? ", bluestein" : "",
+             cls_name(ops[axes[i]].cls), axes[i], ps.args.N,
+             ps.args.bluestein ? ", bluestein" : (ps.args.fast ? ", pow2" : ""),
              lay[i].mo[0], lay[i].mo[1], lay[i].mo[2], lay[i + 1].mo[0], lay[i + 1].mo[1], lay[i + 1].mo[2],
              ps.args.B);
     chain.desc += buf;

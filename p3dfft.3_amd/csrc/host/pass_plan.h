@@ -41,8 +41,11 @@ struct PassDesc {
   unsigned int nblocks;
   bool inplace_ok;
   bool deriv_capable;  // complex output along a transformed axis
-  std::vector<cld> tw, chirp, chirp_hat, pre, post;
+  std::vector<cld> tw, chirp, chirp_hat, pre, post, wN;
 };
+
+// Disable the power-of-two path (tests compare both paths).
+void set_fast_path(bool enable);
 
 struct PassChain {
   std::vector<PassDesc> passes;

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include "fft_core.h"
+#include "fft_fast.h"
 
 namespace p3e {
 
@@ -27,6 +28,38 @@ __global__ void __launch_bounds__(256) pass_kernel(PassArgs p) {
   cx<T> *bufA = reinterpret_cast<cx<T> *>(smem_raw);
   cx<T> *bufB = bufA + (size_t)p.B * p.lpad;
   run_tile<T>(p, blockIdx.x, bufA, bufB, threadIdx.x, blockDim.x, DevBarrier());
+}
+
+// Power-of-two path: one LDS buffer, P3E_FAST_E elements per thread.
+template <typename T>
+__global__ void __launch_bounds__(1024) pass_kernel_fast(PassArgs p) {
+  extern __shared__ unsigned char smem_raw[];
+  cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
+  uint32_t f;
+  int e0, nb;
+  fast_tile_coords(p, blockIdx.x, &f, &e0, &nb);
+  const int tid = threadIdx.x, nth = blockDim.x;
+  fast_load<T>(p, f, e0, nb, buf, tid, nth);
+  __syncthreads();
+  if (p.cls == CLS_C2R) {
+    fast_c2r_merge<T>(p, nb, buf, tid, nth);
+    __syncthreads();
+  }
+  const cx<T> *tw = reinterpret_cast<const cx<T> *>(p.tw);
+  const int s = (p.cls == CLS_R2C) ? -1 : p.sign;
+  cx<T> v[P3E_FAST_E];
+  int lines[P3E_FAST_E], js[P3E_FAST_E];
+  int Ns = 1, logNs = 0;
+  for (int i = 0; i < p.nrad; i++) {
+    const int R = p.radix[i];
+    fast_phase<T>(0, R, buf, p, nb, Ns, logNs, s, tw, tid, nth, v, lines, js);
+    __syncthreads();
+    fast_phase<T>(1, R, buf, p, nb, Ns, logNs, s, tw, tid, nth, v, lines, js);
+    __syncthreads();
+    Ns *= R;
+    logNs += (R == 2) ? 1 : (R == 4) ? 2 : (R == 8) ? 3 : 4;
+  }
+  fast_store<T>(p, f, e0, nb, buf, tid, nth);
 }
 
 // ---------------------------------------------------------------- compute_deriv
@@ -55,6 +88,20 @@ __global__ void deriv_kernel(const cx<T> *in, cx<T> *out, long long n, int s0, i
 extern "C" int p3e_launch_pass(const p3e::PassArgs *a, int prec, size_t lds_bytes, int threads,
                                unsigned int nblocks, hipStream_t st) {
   if (nblocks == 0) return 0;
+  if (a->fast) {
+    if (prec == 8) {
+      if (lds_bytes > 65536)
+        (void)hipFuncSetAttribute((const void *)p3e::pass_kernel_fast<double>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+      hipLaunchKernelGGL(p3e::pass_kernel_fast<double>, dim3(nblocks), dim3(threads), lds_bytes, st, *a);
+    } else {
+      if (lds_bytes > 65536)
+        (void)hipFuncSetAttribute((const void *)p3e::pass_kernel_fast<float>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+      hipLaunchKernelGGL(p3e::pass_kernel_fast<float>, dim3(nblocks), dim3(threads), lds_bytes, st, *a);
+    }
+    return (int)hipGetLastError();
+  }
   if (prec == 8) {
     if (lds_bytes > 65536)
       (void)hipFuncSetAttribute((const void *)p3e::pass_kernel<double>,

@@ -67,6 +67,17 @@ struct PassArgs {
   int has_scale;
   double scale;
   FastDiv div_nin, div_nout, div_ntile;
+  // power-of-two path (fast != 0): one LDS buffer, register-staged passes,
+  // R2C/C2R through a half-length complex FFT of the packed real line
+  int fast;
+  int M, logM;       // complex FFT length of the fast path (N, or N/2 for R2C/C2R)
+  int lp;            // LDS pitch of one line in the fast path (padded)
+  const void *wN;    // R2C/C2R: M+1 entries exp(-2 pi i k / N)
 };
+
+// elements held in registers per thread by the fast path
+#define P3E_FAST_E 16
+// padded LDS index of the fast path: one spare slot every 16 elements
+#define P3E_PAD(i) ((i) + ((i) >> 4))
 
 }  // namespace p3e

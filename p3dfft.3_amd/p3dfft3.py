@@ -205,3 +205,9 @@ def profile_read(plan, npass=8):
     tot = ctypes.c_double()
     n = L.p3dfft_hip_profile_read(plan, avg, npass, ctypes.byref(tot))
     return n, list(avg)[:max(0, npasses(plan))], tot.value
+
+
+def set_fast_path(enable=True):
+    """Plans made afterwards use the power-of-two kernel where it applies
+    (default) or always the generic kernel."""
+    lib().p3dfft_hip_set_fast_path(int(bool(enable)))

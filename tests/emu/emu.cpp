@@ -10,6 +10,7 @@
 
 #include "../../p3dfft.3_amd/csrc/host/pass_plan.h"
 #include "../../p3dfft.3_amd/csrc/kernels/fft_core.h"
+#include "../../p3dfft.3_amd/csrc/kernels/fft_fast.h"
 
 using namespace p3e;
 
@@ -28,7 +29,7 @@ template <typename T> static std::vector<T> tab(const std::vector<cld> &v) {
 
 template <typename T> static int run_pass(const PassDesc &pd, const void *in, void *out, int deriv) {
   std::vector<T> tw = tab<T>(pd.tw), ch = tab<T>(pd.chirp), chh = tab<T>(pd.chirp_hat), pre = tab<T>(pd.pre),
-                 post = tab<T>(pd.post);
+                 post = tab<T>(pd.post), wn = tab<T>(pd.wN);
   PassArgs a = pd.args;
   a.in = in;
   a.out = out;
@@ -37,7 +38,38 @@ template <typename T> static int run_pass(const PassDesc &pd, const void *in, vo
   a.chirp_hat = chh.empty() ? nullptr : chh.data();
   a.pre = pre.empty() ? nullptr : pre.data();
   a.post = post.empty() ? nullptr : post.data();
+  a.wN = wn.empty() ? nullptr : wn.data();
   a.deriv = deriv;
+  if (a.fast) {
+    // every thread of the workgroup is emulated: per-thread registers are
+    // kept across the barrier between the read and write phase of a pass
+    const int nth = pd.threads;
+    std::vector<cx<T>> buf((size_t)a.B * a.lp + a.B);
+    std::vector<cx<T>> v((size_t)nth * P3E_FAST_E);
+    std::vector<int> lines((size_t)nth * P3E_FAST_E), js((size_t)nth * P3E_FAST_E);
+    const cx<T> *twp = reinterpret_cast<const cx<T> *>(a.tw);
+    const int s = (a.cls == CLS_R2C) ? -1 : a.sign;
+    for (unsigned int t = 0; t < pd.nblocks; t++) {
+      for (auto &x : buf) x = cx<T>{(T)1e30, (T)-1e30};
+      uint32_t f;
+      int e0, nb;
+      fast_tile_coords(a, t, &f, &e0, &nb);
+      fast_load<T>(a, f, e0, nb, buf.data(), 0, 1);
+      if (a.cls == CLS_C2R) fast_c2r_merge<T>(a, nb, buf.data(), 0, 1);
+      int Ns = 1, logNs = 0;
+      for (int i = 0; i < a.nrad; i++) {
+        const int R = a.radix[i];
+        for (int ph = 0; ph < 2; ph++)
+          for (int tid = 0; tid < nth; tid++)
+            fast_phase<T>(ph, R, buf.data(), a, nb, Ns, logNs, s, twp, tid, nth, &v[(size_t)tid * P3E_FAST_E],
+                          &lines[(size_t)tid * P3E_FAST_E], &js[(size_t)tid * P3E_FAST_E]);
+        Ns *= R;
+        logNs += (R == 2) ? 1 : (R == 4) ? 2 : (R == 8) ? 3 : 4;
+      }
+      fast_store<T>(a, f, e0, nb, buf.data(), 0, 1);
+    }
+    return 0;
+  }
   std::vector<cx<T>> A((size_t)a.B * a.lpad), B((size_t)a.B * a.lpad);
   for (unsigned int t = 0; t < pd.nblocks; t++) {
     // poison LDS between tiles so stale data cannot hide a missing write
@@ -97,6 +129,10 @@ int emu_exec(int prec, const int *cls, const int *sign, const int *kind, const i
 }
 
 const char *emu_error(void) { return last_error(); }
+
+// 1: plan with the power-of-two path where eligible (the product default);
+// 0: generic path only.
+void emu_set_fast(int enable) { set_fast_path(enable != 0); }
 
 // Buffer assignment of a plan, for tests: dst[i] per pass, scratch sizes.
 int emu_assign(int prec, const int *cls, const int *sign, const int *kind, const int *idt, const int *odt,
