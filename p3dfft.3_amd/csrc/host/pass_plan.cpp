@@ -142,7 +142,8 @@ static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 // Switch an already described pass to the power-of-two path when it is a
 // C2C, R2C or C2R of power-of-two complex length M in [16, 4096] (M = N/2 for
 // the real transforms).  Tile: B lines, B as large as the LDS budget and
-// 1024 threads allow (up to 64), P3E_FAST_E elements per thread.
+// P3E_FAST_MAX_THREADS threads allow (up to 64), P3E_FAST_E elements per
+// thread.
 static void setup_fast(int prec, const OpSpec &op, PassDesc &ps) {
   PassArgs &a = ps.args;
   if (!g_fast_enabled || a.bluestein) return;
@@ -150,25 +151,24 @@ static void setup_fast(int prec, const OpSpec &op, PassDesc &ps) {
   if (op.cls == CLS_C2C) M = a.N;
   else if ((op.cls == CLS_R2C || op.cls == CLS_C2R) && a.N % 2 == 0) M = a.N / 2;
   else return;
-  if (!is_pow2(M) || M < 16 || M > 4096) return;
-  std::vector<int> rad;
-  factor(M, rad);
-  for (int r : rad)
-    if (P3E_FAST_E % r) return;
+  int rad[P3E_MAX_RADIX_PASSES];
+  const int nrad = fast_radices(M, rad);
+  if (nrad == 0) return;
   const size_t esz = 2 * (size_t)prec;
   const int lp = M + M / 16;
+  const long maxth = P3E_FAST_MAX_THREADS;
   int B = 1;
   const int Bmax = std::min(64, next_pow2(a.ne));
   auto lds = [&](int b) { return ((size_t)b * lp + b) * esz; };
-  while (B * 2 <= Bmax && lds(B * 2) <= LDS_BUDGET && (long)M * B * 2 / P3E_FAST_E <= 1024) B *= 2;
-  if (lds(B) > LDS_MAX || (long)M * B / P3E_FAST_E > 1024) return;
+  while (B * 2 <= Bmax && lds(B * 2) <= LDS_BUDGET && (long)M * B * 2 / P3E_FAST_E <= maxth) B *= 2;
+  if (lds(B) > LDS_MAX || (long)M * B / P3E_FAST_E > maxth) return;
   a.fast = 1;
   a.M = M;
   a.logM = 0;
   while ((1 << a.logM) < M) a.logM++;
   a.lp = lp;
-  a.nrad = (int)rad.size();
-  for (size_t i = 0; i < rad.size(); i++) a.radix[i] = rad[i];
+  a.nrad = nrad;
+  for (int i = 0; i < nrad; i++) a.radix[i] = rad[i];
   a.L = M;
   a.zero_fill = 0;
   a.B = B;

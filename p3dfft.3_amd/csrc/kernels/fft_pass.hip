@@ -30,36 +30,51 @@ __global__ void __launch_bounds__(256) pass_kernel(PassArgs p) {
   run_tile<T>(p, blockIdx.x, bufA, bufB, threadIdx.x, blockDim.x, DevBarrier());
 }
 
-// Power-of-two path: one LDS buffer, P3E_FAST_E elements per thread.
-template <typename T>
-__global__ void __launch_bounds__(1024) pass_kernel_fast(PassArgs p) {
+// One radix pass of the power-of-two path for this thread: read half,
+// barrier, write half, barrier; the thread's values live in registers.
+template <typename T> struct DevPassRun {
+  int tid, nth;
+  template <class PS> __device__ void operator()(PS &ps) const {
+    cx<T> v[P3E_FAST_E];
+    ps.read(tid, nth, v);
+    __syncthreads();
+    ps.write(tid, nth, v);
+    __syncthreads();
+  }
+};
+
+// Power-of-two path: complex length M at compile time, one LDS buffer.
+template <typename T, int M>
+__global__ void __launch_bounds__(P3E_FAST_MAX_THREADS) pass_kernel_pow2(PassArgs p) {
   extern __shared__ unsigned char smem_raw[];
   cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
-  uint32_t f;
-  int e0, nb;
-  fast_tile_coords(p, blockIdx.x, &f, &e0, &nb);
-  const int tid = threadIdx.x, nth = blockDim.x;
-  fast_load<T>(p, f, e0, nb, buf, tid, nth);
-  __syncthreads();
-  if (p.cls == CLS_C2R) {
-    fast_c2r_merge<T>(p, nb, buf, tid, nth);
-    __syncthreads();
+  DevPassRun<T> run{(int)threadIdx.x, (int)blockDim.x};
+  fast_tile<T, M>(p, blockIdx.x, buf, threadIdx.x, blockDim.x, run, DevBarrier());
+}
+
+template <typename T, int M>
+static int launch_pow2(const PassArgs *a, size_t lds_bytes, int threads, unsigned int nblocks, hipStream_t st) {
+  if (lds_bytes > 65536)
+    (void)hipFuncSetAttribute((const void *)pass_kernel_pow2<T, M>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds_bytes);
+  hipLaunchKernelGGL((pass_kernel_pow2<T, M>), dim3(nblocks), dim3(threads), lds_bytes, st, *a);
+  return (int)hipGetLastError();
+}
+
+template <typename T>
+static int launch_pow2_any(const PassArgs *a, size_t lds_bytes, int threads, unsigned int nblocks, hipStream_t st) {
+  switch (a->M) {
+    case 16: return launch_pow2<T, 16>(a, lds_bytes, threads, nblocks, st);
+    case 32: return launch_pow2<T, 32>(a, lds_bytes, threads, nblocks, st);
+    case 64: return launch_pow2<T, 64>(a, lds_bytes, threads, nblocks, st);
+    case 128: return launch_pow2<T, 128>(a, lds_bytes, threads, nblocks, st);
+    case 256: return launch_pow2<T, 256>(a, lds_bytes, threads, nblocks, st);
+    case 512: return launch_pow2<T, 512>(a, lds_bytes, threads, nblocks, st);
+    case 1024: return launch_pow2<T, 1024>(a, lds_bytes, threads, nblocks, st);
+    case 2048: return launch_pow2<T, 2048>(a, lds_bytes, threads, nblocks, st);
+    case 4096: return launch_pow2<T, 4096>(a, lds_bytes, threads, nblocks, st);
   }
-  const cx<T> *tw = reinterpret_cast<const cx<T> *>(p.tw);
-  const int s = (p.cls == CLS_R2C) ? -1 : p.sign;
-  cx<T> v[P3E_FAST_E];
-  int lines[P3E_FAST_E], js[P3E_FAST_E];
-  int Ns = 1, logNs = 0;
-  for (int i = 0; i < p.nrad; i++) {
-    const int R = p.radix[i];
-    fast_phase<T>(0, R, buf, p, nb, Ns, logNs, s, tw, tid, nth, v, lines, js);
-    __syncthreads();
-    fast_phase<T>(1, R, buf, p, nb, Ns, logNs, s, tw, tid, nth, v, lines, js);
-    __syncthreads();
-    Ns *= R;
-    logNs += (R == 2) ? 1 : (R == 4) ? 2 : (R == 8) ? 3 : 4;
-  }
-  fast_store<T>(p, f, e0, nb, buf, tid, nth);
+  return (int)hipErrorInvalidValue;
 }
 
 // ---------------------------------------------------------------- compute_deriv
@@ -89,18 +104,9 @@ extern "C" int p3e_launch_pass(const p3e::PassArgs *a, int prec, size_t lds_byte
                                unsigned int nblocks, hipStream_t st) {
   if (nblocks == 0) return 0;
   if (a->fast) {
-    if (prec == 8) {
-      if (lds_bytes > 65536)
-        (void)hipFuncSetAttribute((const void *)p3e::pass_kernel_fast<double>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
-      hipLaunchKernelGGL(p3e::pass_kernel_fast<double>, dim3(nblocks), dim3(threads), lds_bytes, st, *a);
-    } else {
-      if (lds_bytes > 65536)
-        (void)hipFuncSetAttribute((const void *)p3e::pass_kernel_fast<float>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
-      hipLaunchKernelGGL(p3e::pass_kernel_fast<float>, dim3(nblocks), dim3(threads), lds_bytes, st, *a);
-    }
-    return (int)hipGetLastError();
+    if (threads < 1 || threads > P3E_FAST_MAX_THREADS) return (int)hipErrorInvalidValue;
+    return prec == 8 ? p3e::launch_pow2_any<double>(a, lds_bytes, threads, nblocks, st)
+                     : p3e::launch_pow2_any<float>(a, lds_bytes, threads, nblocks, st);
   }
   if (prec == 8) {
     if (lds_bytes > 65536)

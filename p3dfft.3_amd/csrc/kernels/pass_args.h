@@ -77,6 +77,26 @@ struct PassArgs {
 
 // elements held in registers per thread by the fast path
 #define P3E_FAST_E 16
+// threads per workgroup of the fast path, at most
+#define P3E_FAST_MAX_THREADS 512
+
+// Radix sequence of the power-of-two path for each complex length M (the
+// kernel's compile-time sequences in fft_fast.h are the same); 0 if M is not
+// supported.
+static inline int fast_radices(int M, int *rad) {
+  switch (M) {
+    case 16: rad[0] = 16; return 1;
+    case 32: rad[0] = 8; rad[1] = 4; return 2;
+    case 64: rad[0] = 8; rad[1] = 8; return 2;
+    case 128: rad[0] = 16; rad[1] = 8; return 2;
+    case 256: rad[0] = 16; rad[1] = 16; return 2;
+    case 512: rad[0] = 8; rad[1] = 8; rad[2] = 8; return 3;
+    case 1024: rad[0] = 16; rad[1] = 8; rad[2] = 8; return 3;
+    case 2048: rad[0] = 16; rad[1] = 16; rad[2] = 8; return 3;
+    case 4096: rad[0] = 16; rad[1] = 16; rad[2] = 16; return 3;
+  }
+  return 0;
+}
 // padded LDS index of the fast path: one spare slot every 16 elements
 #define P3E_PAD(i) ((i) + ((i) >> 4))
 

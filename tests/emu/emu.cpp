@@ -27,6 +27,27 @@ template <typename T> static std::vector<T> tab(const std::vector<cld> &v) {
   return h;
 }
 
+// Every thread of the workgroup is emulated for a radix pass: all read
+// halves, then all write halves, each thread keeping its own registers.
+template <typename T> struct HostPassRun {
+  int nth;
+  std::vector<cx<T>> v;
+  template <class PS> void operator()(PS &ps) {
+    for (int tid = 0; tid < nth; tid++) ps.read(tid, nth, &v[(size_t)tid * P3E_FAST_E]);
+    for (int tid = 0; tid < nth; tid++) ps.write(tid, nth, &v[(size_t)tid * P3E_FAST_E]);
+  }
+};
+
+template <typename T, int M> static int run_pow2(const PassDesc &pd, const PassArgs &a) {
+  HostPassRun<T> run{pd.threads, std::vector<cx<T>>((size_t)pd.threads * P3E_FAST_E)};
+  std::vector<cx<T>> buf((size_t)a.B * a.lp + a.B);
+  for (unsigned int t = 0; t < pd.nblocks; t++) {
+    for (auto &x : buf) x = cx<T>{(T)1e30, (T)-1e30};
+    fast_tile<T, M>(a, t, buf.data(), 0, 1, run, HostBarrier());
+  }
+  return 0;
+}
+
 template <typename T> static int run_pass(const PassDesc &pd, const void *in, void *out, int deriv) {
   std::vector<T> tw = tab<T>(pd.tw), ch = tab<T>(pd.chirp), chh = tab<T>(pd.chirp_hat), pre = tab<T>(pd.pre),
                  post = tab<T>(pd.post), wn = tab<T>(pd.wN);
@@ -41,34 +62,18 @@ template <typename T> static int run_pass(const PassDesc &pd, const void *in, vo
   a.wN = wn.empty() ? nullptr : wn.data();
   a.deriv = deriv;
   if (a.fast) {
-    // every thread of the workgroup is emulated: per-thread registers are
-    // kept across the barrier between the read and write phase of a pass
-    const int nth = pd.threads;
-    std::vector<cx<T>> buf((size_t)a.B * a.lp + a.B);
-    std::vector<cx<T>> v((size_t)nth * P3E_FAST_E);
-    std::vector<int> lines((size_t)nth * P3E_FAST_E), js((size_t)nth * P3E_FAST_E);
-    const cx<T> *twp = reinterpret_cast<const cx<T> *>(a.tw);
-    const int s = (a.cls == CLS_R2C) ? -1 : a.sign;
-    for (unsigned int t = 0; t < pd.nblocks; t++) {
-      for (auto &x : buf) x = cx<T>{(T)1e30, (T)-1e30};
-      uint32_t f;
-      int e0, nb;
-      fast_tile_coords(a, t, &f, &e0, &nb);
-      fast_load<T>(a, f, e0, nb, buf.data(), 0, 1);
-      if (a.cls == CLS_C2R) fast_c2r_merge<T>(a, nb, buf.data(), 0, 1);
-      int Ns = 1, logNs = 0;
-      for (int i = 0; i < a.nrad; i++) {
-        const int R = a.radix[i];
-        for (int ph = 0; ph < 2; ph++)
-          for (int tid = 0; tid < nth; tid++)
-            fast_phase<T>(ph, R, buf.data(), a, nb, Ns, logNs, s, twp, tid, nth, &v[(size_t)tid * P3E_FAST_E],
-                          &lines[(size_t)tid * P3E_FAST_E], &js[(size_t)tid * P3E_FAST_E]);
-        Ns *= R;
-        logNs += (R == 2) ? 1 : (R == 4) ? 2 : (R == 8) ? 3 : 4;
-      }
-      fast_store<T>(a, f, e0, nb, buf.data(), 0, 1);
+    switch (a.M) {
+      case 16: return run_pow2<T, 16>(pd, a);
+      case 32: return run_pow2<T, 32>(pd, a);
+      case 64: return run_pow2<T, 64>(pd, a);
+      case 128: return run_pow2<T, 128>(pd, a);
+      case 256: return run_pow2<T, 256>(pd, a);
+      case 512: return run_pow2<T, 512>(pd, a);
+      case 1024: return run_pow2<T, 1024>(pd, a);
+      case 2048: return run_pow2<T, 2048>(pd, a);
+      case 4096: return run_pow2<T, 4096>(pd, a);
     }
-    return 0;
+    return -1;
   }
   std::vector<cx<T>> A((size_t)a.B * a.lpad), B((size_t)a.B * a.lpad);
   for (unsigned int t = 0; t < pd.nblocks; t++) {

@@ -346,9 +346,12 @@ def test_config2_512cubed_roundtrip_and_derivative(prec):
     err = np.abs(got - ana).max()
     rel = np.linalg.norm(got - ana) / np.linalg.norm(ana)
     log("  d/dx max abs error", err, "rel L2", rel)
-    # FP32: i*k (|k| up to n/2) amplifies the spectrum's rounding error, so
-    # the bar is the relative L2 error of the derivative field
+    # FP32: i*k (|k| up to n/2 = 256) amplifies every rounding error in the
+    # spectrum, starting with the FP32 rounding of the input (~6e-8 relative,
+    # spread over all modes, k_rms ~ 150): the derivative field cannot be
+    # closer than ~1e-5 to the exact one.  The bar is its relative L2 error
+    # at 5e-5 (FP64: max abs 1e-12).
     if prec == 8:
         assert err < 1e-12
     else:
-        assert rel < 1e-5
+        assert rel < 5e-5
