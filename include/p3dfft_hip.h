@@ -48,6 +48,25 @@ int p3dfft_hip_profile_read(int plan, double *avg_ms, int cap, double *total_ms)
  * For testing both kernels on the same transforms. */
 void p3dfft_hip_set_fast_path(int enable);
 
+/* Batches of independent 3D transforms of possibly different sizes, all of
+ * the same three 1D types and memory orders.  Item i has logical input dims
+ * dims_in[3i..3i+2] and output dims dims_out[3i..3i+2].  Every stage of all
+ * items runs in one launch of a grid-stride kernel over their tiles.
+ * tune != 0 lets an autotuner pick the tile size and radix order of the
+ * non-power-of-two stages (results cached in the file named by
+ * P3DFFT_HIP_TUNE_CACHE, default $HOME/.cache/p3dfft3_hip_tune.txt).
+ * Returns a batch handle, or -1. */
+int p3dfft_hip_batch_plan(int nitems, const int *dims_in, const int *dims_out, const int types[3],
+                          const int mo_in[3], const int mo_out[3], int tune);
+/* in[i]/out[i]: device pointers of item i (in[i] == out[i] runs in place when
+ * the stages allow it).  sync != 0 waits for completion. */
+int p3dfft_hip_batch_exec(int bplan, void *const *in, void *const *out, void *stream, int sync);
+/* enable != 0: the first exec captures the launches in a hipGraph, later
+ * execs replay it (pointers may change between calls). */
+int p3dfft_hip_batch_set_graph(int bplan, int enable);
+int p3dfft_hip_batch_describe(int bplan, char *buf, int len);
+int p3dfft_hip_batch_destroy(int bplan);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,413 @@
+// Batched execution: many independent transforms, one step-kernel launch per
+// stage for all of them (optionally replayed from a hipGraph), with tile
+// shapes and radix orders of the generic-path passes chosen by an autotuner.
+#include "batch.h"
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/stat.h>
+
+#include <algorithm>
+#include <map>
+#include <string>
+
+extern "C" int p3e_launch_step(const void *table, const void *work, int nwork, int prec, size_t lds,
+                               unsigned int grid, hipStream_t st);
+extern "C" int p3e_step_occupancy(int prec, size_t lds);
+
+namespace p3e {
+
+#define BCHK(x)                                                  \
+  do {                                                           \
+    hipError_t e_ = (x);                                         \
+    if (e_ != hipSuccess) {                                      \
+      set_error("%s failed: %s", #x, hipGetErrorString(e_));     \
+      return -1;                                                 \
+    }                                                            \
+  } while (0)
+
+template <typename T> static void *upload_t(const std::vector<cld> &v, std::vector<void *> &owned) {
+  std::vector<T> h(2 * v.size());
+  for (size_t i = 0; i < v.size(); i++) {
+    h[2 * i] = (T)v[i].real();
+    h[2 * i + 1] = (T)v[i].imag();
+  }
+  void *d = nullptr;
+  if (hipMalloc(&d, h.size() * sizeof(T)) != hipSuccess) return nullptr;
+  if (hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(d);
+    return nullptr;
+  }
+  owned.push_back(d);
+  return d;
+}
+
+static bool upload_pass_tables(int prec, PassDesc &pd, std::vector<void *> &owned) {
+  PassArgs &a = pd.args;
+  const std::vector<cld> *src[6] = {&pd.tw, &pd.chirp, &pd.chirp_hat, &pd.pre, &pd.post, &pd.wN};
+  const void **dst[6] = {&a.tw, &a.chirp, &a.chirp_hat, &a.pre, &a.post, &a.wN};
+  for (int i = 0; i < 6; i++) {
+    *dst[i] = nullptr;
+    if (src[i]->empty()) continue;
+    void *d = prec == 8 ? upload_t<double>(*src[i], owned) : upload_t<float>(*src[i], owned);
+    if (!d) {
+      set_error("hipMalloc/hipMemcpy failed for a constant table");
+      return false;
+    }
+    *dst[i] = d;
+  }
+  return true;
+}
+
+static unsigned int persistent_grid(int prec, size_t lds, int nwork) {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0;
+    cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+      int n = 0;
+      if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0) cus = n;
+    }
+  }
+  int per_cu = p3e_step_occupancy(prec, lds);
+  if (per_cu < 1) per_cu = 1;
+  long g = (long)cus * per_cu;
+  if (g > nwork) g = nwork;
+  return (unsigned int)std::max(1L, g);
+}
+
+// ------------------------------------------------------------------ autotuner
+struct TuneChoice {
+  int B;
+  std::vector<int> rad;
+};
+static std::map<std::string, TuneChoice> g_tune;
+static bool g_tune_loaded = false;
+
+static std::string cache_path() {
+  const char *p = getenv("P3DFFT_HIP_TUNE_CACHE");
+  if (p && *p) return p;
+  const char *h = getenv("HOME");
+  return std::string(h ? h : "/tmp") + "/.cache/p3dfft3_hip_tune.txt";
+}
+
+static void load_cache() {
+  if (g_tune_loaded) return;
+  g_tune_loaded = true;
+  FILE *f = fopen(cache_path().c_str(), "r");
+  if (!f) return;
+  char key[256], rads[256];
+  int B;
+  while (fscanf(f, "%255s %d %255s", key, &B, rads) == 3) {
+    TuneChoice c;
+    c.B = B;
+    for (char *t = strtok(rads, ","); t; t = strtok(nullptr, ",")) c.rad.push_back(atoi(t));
+    g_tune[key] = c;
+  }
+  fclose(f);
+}
+
+static void save_cache() {
+  std::string path = cache_path();
+  const size_t slash = path.find_last_of('/');
+  if (slash != std::string::npos && slash > 0) (void)mkdir(path.substr(0, slash).c_str(), 0755);
+  FILE *f = fopen(path.c_str(), "w");
+  if (!f) return;
+  for (auto &kv : g_tune) {
+    fprintf(f, "%s %d ", kv.first.c_str(), kv.second.B);
+    for (size_t i = 0; i < kv.second.rad.size(); i++) fprintf(f, "%s%d", i ? "," : "", kv.second.rad[i]);
+    fprintf(f, "\n");
+  }
+  fclose(f);
+}
+
+int tune_cache_size() {
+  load_cache();
+  return (int)g_tune.size();
+}
+
+static std::string tune_key(int prec, const PassDesc &pd) {
+  const PassArgs &a = pd.args;
+  char k[256];
+  snprintf(k, sizeof(k), "gfx950:p%d:c%d:k%d:N%d:W%d:L%d:o%d%d:e%d", prec, a.cls, a.r2r_kind, a.N, a.W, a.L,
+           a.load_order, a.store_order, a.ne);
+  return k;
+}
+
+// Time one candidate: all tiles of the pass through the step kernel.
+static float time_candidate(int prec, PassDesc &pd, void *din, void *dout, PassArgs *d_tab, WorkItem *d_work,
+                            int maxwork, hipStream_t st) {
+  PassArgs a = pd.args;
+  a.in = din;
+  a.out = dout;
+  a.deriv = 0;
+  const int nw = std::min<int>((int)pd.nblocks, maxwork);
+  std::vector<WorkItem> w(nw);
+  for (int t = 0; t < nw; t++) w[t] = WorkItem{0, (uint32_t)t};
+  if (hipMemcpy(d_tab, &a, sizeof(a), hipMemcpyHostToDevice) != hipSuccess) return 1e30f;
+  if (hipMemcpy(d_work, w.data(), sizeof(WorkItem) * nw, hipMemcpyHostToDevice) != hipSuccess) return 1e30f;
+  const unsigned int grid = persistent_grid(prec, pd.lds_bytes, nw);
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  float best = 1e30f;
+  for (int rep = 0; rep < 4; rep++) {
+    (void)hipEventRecord(e0, st);
+    int rc = p3e_launch_step(d_tab, d_work, nw, prec, pd.lds_bytes, grid, st);
+    (void)hipEventRecord(e1, st);
+    (void)hipEventSynchronize(e1);
+    if (rc) {
+      best = 1e30f;
+      break;
+    }
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (rep > 0) best = std::min(best, ms);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return best;
+}
+
+static void tune_pass(int prec, PassDesc &pd, hipStream_t st) {
+  PassArgs &a = pd.args;
+  if (a.fast || a.cls == CLS_EMPTY) return;
+  load_cache();
+  const std::string key = tune_key(prec, pd);
+  auto it = g_tune.find(key);
+  if (it != g_tune.end()) {
+    PassDesc t = pd;
+    if (retune_pass(prec, t, it->second.B, it->second.rad.data(), (int)it->second.rad.size())) pd = t;
+    return;
+  }
+  void *din = nullptr, *dout = nullptr;
+  PassArgs *d_tab = nullptr;
+  WorkItem *d_work = nullptr;
+  const int maxwork = 4096;
+  if (hipMalloc(&din, std::max(pd.in_bytes, (size_t)16)) != hipSuccess ||
+      hipMalloc(&dout, std::max(pd.out_bytes, (size_t)16)) != hipSuccess ||
+      hipMalloc(&d_tab, sizeof(PassArgs)) != hipSuccess ||
+      hipMalloc(&d_work, sizeof(WorkItem) * maxwork) != hipSuccess) {
+    (void)hipGetLastError();
+  } else {
+    (void)hipMemsetAsync(din, 0, pd.in_bytes, st);
+    std::vector<std::vector<int>> rads = radix_candidates(a.L, 4);
+    TuneChoice best{a.B, std::vector<int>(a.radix, a.radix + a.nrad)};
+    float best_ms = time_candidate(prec, pd, din, dout, d_tab, d_work, maxwork, st);
+    for (int B : {4, 8, 16, 32}) {
+      for (const auto &r : rads) {
+        PassDesc t = pd;
+        if (!retune_pass(prec, t, B, r.data(), (int)r.size())) continue;
+        if (t.lds_bytes > 96 * 1024) continue;
+        const float ms = time_candidate(prec, t, din, dout, d_tab, d_work, maxwork, st);
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = TuneChoice{B, r};
+        }
+      }
+    }
+    g_tune[key] = best;
+    save_cache();
+    PassDesc t = pd;
+    if (retune_pass(prec, t, best.B, best.rad.data(), (int)best.rad.size())) pd = t;
+  }
+  if (din) (void)hipFree(din);
+  if (dout) (void)hipFree(dout);
+  if (d_tab) (void)hipFree(d_tab);
+  if (d_work) (void)hipFree(d_work);
+}
+
+// ------------------------------------------------------------------ batch
+BatchPlan *batch_create(int prec, int nitems, const OpSpec ops[3], const int order[3], const Layout *in,
+                        const Layout *out, bool tune) {
+  if (nitems < 1) {
+    set_error("batch: no items");
+    return nullptr;
+  }
+  BatchPlan *b = new BatchPlan();
+  b->prec = prec;
+  b->nitems = nitems;
+  b->h_table = nullptr;
+  b->d_table = nullptr;
+  b->scratch = nullptr;
+  b->scratch_bytes = 0;
+  b->use_graph = false;
+  b->graph_ok = false;
+  b->graph = nullptr;
+  b->gexec = nullptr;
+  b->table_pending = false;
+  b->lds = 0;
+  b->grid = 1;
+  (void)hipEventCreateWithFlags(&b->table_done, hipEventDisableTiming);
+  b->chains.resize(nitems);
+  b->lin.assign(in, in + nitems);
+  b->lout.assign(out, out + nitems);
+  hipStream_t st = nullptr;
+  for (int i = 0; i < nitems; i++) {
+    if (!plan_chain(prec, ops, order, in[i], out[i], b->chains[i], P3E_STEP_THREADS)) {
+      batch_destroy(b);
+      return nullptr;
+    }
+    if (b->chains[i].copy_only || b->chains[i].passes.empty()) {
+      set_error("batch: item %d has no transform stage", i);
+      batch_destroy(b);
+      return nullptr;
+    }
+    if (i > 0 && b->chains[i].passes.size() != b->chains[0].passes.size()) {
+      set_error("batch: items differ in their number of stages");
+      batch_destroy(b);
+      return nullptr;
+    }
+  }
+  b->nstages = (int)b->chains[0].passes.size();
+  // constant tables, tuning
+  for (int i = 0; i < nitems; i++)
+    for (PassDesc &pd : b->chains[i].passes) {
+      if (tune) tune_pass(prec, pd, st);
+      if (!upload_pass_tables(prec, pd, b->owned)) {
+        batch_destroy(b);
+        return nullptr;
+      }
+    }
+  // scratch: worst case over the in != out and in == out assignments
+  b->scratch_off.resize(2 * nitems);
+  b->scratch_sz.resize(2 * nitems);
+  size_t off = 0;
+  for (int i = 0; i < nitems; i++) {
+    size_t need_a[2], need_b[2];
+    int dst[8];
+    bool cb;
+    const size_t outb = layout_elems(out[i]) * elem_bytes(prec, out[i].dt);
+    assign_buffers(b->chains[i].passes, outb, false, dst, need_a, &cb);
+    assign_buffers(b->chains[i].passes, outb, true, dst, need_b, &cb);
+    for (int s = 0; s < 2; s++) {
+      const size_t n = (std::max(need_a[s], need_b[s]) + 255) & ~(size_t)255;
+      b->scratch_off[2 * i + s] = off;
+      b->scratch_sz[2 * i + s] = n;
+      off += n;
+    }
+  }
+  b->scratch_bytes = off;
+  if (off && hipMalloc(&b->scratch, off) != hipSuccess) {
+    set_error("batch: hipMalloc of %zu scratch bytes failed", off);
+    batch_destroy(b);
+    return nullptr;
+  }
+  // stage-major table and work lists
+  const size_t ntab = (size_t)b->nstages * nitems;
+  b->table.resize(ntab);
+  for (int s = 0; s < b->nstages; s++)
+    for (int i = 0; i < nitems; i++) b->table[(size_t)s * nitems + i] = b->chains[i].passes[s].args;
+  if (hipHostMalloc((void **)&b->h_table, sizeof(PassArgs) * ntab, hipHostMallocDefault) != hipSuccess ||
+      hipMalloc((void **)&b->d_table, sizeof(PassArgs) * ntab) != hipSuccess) {
+    set_error("batch: allocation of the pass table failed");
+    batch_destroy(b);
+    return nullptr;
+  }
+  b->nwork.resize(b->nstages);
+  b->d_work.resize(b->nstages, nullptr);
+  char buf[128];
+  for (int s = 0; s < b->nstages; s++) {
+    std::vector<WorkItem> w;
+    size_t lds = 0;
+    for (int i = 0; i < nitems; i++) {
+      const PassDesc &pd = b->chains[i].passes[s];
+      for (unsigned int t = 0; t < pd.nblocks; t++) w.push_back(WorkItem{(uint32_t)(s * nitems + i), t});
+      lds = std::max(lds, pd.lds_bytes);
+    }
+    b->nwork[s] = (int)w.size();
+    b->lds = std::max(b->lds, lds);
+    if (hipMalloc((void **)&b->d_work[s], sizeof(WorkItem) * std::max<size_t>(1, w.size())) != hipSuccess ||
+        hipMemcpy(b->d_work[s], w.data(), sizeof(WorkItem) * w.size(), hipMemcpyHostToDevice) != hipSuccess) {
+      set_error("batch: work list upload failed");
+      batch_destroy(b);
+      return nullptr;
+    }
+    snprintf(buf, sizeof(buf), "%sstage %d: %d tiles, LDS %zu B", s ? " | " : "", s, b->nwork[s], lds);
+    b->desc += buf;
+  }
+  b->grid = 0;
+  return b;
+}
+
+int batch_exec(BatchPlan *b, void *const *in, void *const *out, hipStream_t st, bool sync) {
+  if (!b) {
+    set_error("batch exec on an invalid plan");
+    return -1;
+  }
+  if (b->table_pending) BCHK(hipEventSynchronize(b->table_done));
+  // buffers of every stage of every item
+  for (int i = 0; i < b->nitems; i++) {
+    const std::vector<PassDesc> &ps = b->chains[i].passes;
+    const size_t outb = layout_elems(b->lout[i]) * elem_bytes(b->prec, b->lout[i].dt);
+    int dst[8];
+    size_t need[2];
+    bool copy_back = false;
+    assign_buffers(ps, outb, in[i] == out[i], dst, need, &copy_back);
+    if (copy_back) {
+      set_error("batch: item %d cannot run in place", i);
+      return -1;
+    }
+    const void *src = in[i];
+    for (int s = 0; s < b->nstages; s++) {
+      void *d = out[i];
+      if (dst[s] == BUF_S0 || dst[s] == BUF_S1) {
+        const int k = 2 * i + (dst[s] - BUF_S0);
+        d = (char *)b->scratch + b->scratch_off[k];
+      }
+      PassArgs &a = b->table[(size_t)s * b->nitems + i];
+      a.in = src;
+      a.out = d;
+      src = d;
+    }
+  }
+  memcpy(b->h_table, b->table.data(), sizeof(PassArgs) * b->table.size());
+  if (b->use_graph && b->graph_ok) {
+    BCHK(hipGraphLaunch(b->gexec, st));
+  } else {
+    bool capture = b->use_graph && !b->graph_ok;
+    if (capture) BCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    BCHK(hipMemcpyAsync(b->d_table, b->h_table, sizeof(PassArgs) * b->table.size(), hipMemcpyHostToDevice, st));
+    for (int s = 0; s < b->nstages; s++) {
+      size_t lds = 0;
+      for (int i = 0; i < b->nitems; i++) lds = std::max(lds, b->chains[i].passes[s].lds_bytes);
+      const unsigned int grid = persistent_grid(b->prec, lds, b->nwork[s]);
+      int rc = p3e_launch_step(b->d_table, b->d_work[s], b->nwork[s], b->prec, lds, grid, st);
+      if (rc) {
+        if (capture) {
+          hipGraph_t g;
+          (void)hipStreamEndCapture(st, &g);
+        }
+        set_error("batch step launch failed: %s", hipGetErrorString((hipError_t)rc));
+        return -1;
+      }
+    }
+    if (capture) {
+      BCHK(hipStreamEndCapture(st, &b->graph));
+      BCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+      b->graph_ok = true;
+      BCHK(hipGraphLaunch(b->gexec, st));
+    }
+  }
+  BCHK(hipEventRecord(b->table_done, st));
+  b->table_pending = true;
+  if (sync) BCHK(hipStreamSynchronize(st));
+  return 0;
+}
+
+void batch_destroy(BatchPlan *b) {
+  if (!b) return;
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (void *p : b->owned) (void)hipFree(p);
+  for (WorkItem *w : b->d_work)
+    if (w) (void)hipFree(w);
+  if (b->d_table) (void)hipFree(b->d_table);
+  if (b->h_table) (void)hipHostFree(b->h_table);
+  if (b->scratch) (void)hipFree(b->scratch);
+  if (b->table_done) (void)hipEventDestroy(b->table_done);
+  delete b;
+}
+
+}  // namespace p3e

@@ -144,7 +144,7 @@ static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 // the real transforms).  Tile: B lines, B as large as the LDS budget and
 // P3E_FAST_MAX_THREADS threads allow (up to 64), P3E_FAST_E elements per
 // thread.
-static void setup_fast(int prec, const OpSpec &op, PassDesc &ps) {
+static void setup_fast(int prec, const OpSpec &op, PassDesc &ps, int batch_threads) {
   PassArgs &a = ps.args;
   if (!g_fast_enabled || a.bluestein) return;
   int M;
@@ -156,9 +156,9 @@ static void setup_fast(int prec, const OpSpec &op, PassDesc &ps) {
   if (nrad == 0) return;
   const size_t esz = 2 * (size_t)prec;
   const int lp = M + M / 16;
-  const long maxth = P3E_FAST_MAX_THREADS;
+  const long maxth = batch_threads > 0 ? batch_threads : P3E_FAST_MAX_THREADS;
   int B = 1;
-  const int Bmax = std::min(64, next_pow2(a.ne));
+  const int Bmax = std::min(batch_threads > 0 ? 256 : 64, next_pow2(a.ne));
   auto lds = [&](int b) { return ((size_t)b * lp + b) * esz; };
   while (B * 2 <= Bmax && lds(B * 2) <= LDS_BUDGET && (long)M * B * 2 / P3E_FAST_E <= maxth) B *= 2;
   if (lds(B) > LDS_MAX || (long)M * B / P3E_FAST_E > maxth) return;
@@ -178,6 +178,7 @@ static void setup_fast(int prec, const OpSpec &op, PassDesc &ps) {
   a.div_ntile = make_fastdiv((uint32_t)a.ntile_e);
   ps.lds_bytes = lds(B);
   ps.threads = M * B / P3E_FAST_E;
+  a.nthreads = ps.threads;
   ps.nblocks = (unsigned int)((unsigned long long)a.ntile_e * (unsigned long long)a.nf);
   ps.tw.resize(M);
   for (int t = 0; t < M; t++) ps.tw[t] = expi_pi(-2LL * t, M);
@@ -189,7 +190,8 @@ static void setup_fast(int prec, const OpSpec &op, PassDesc &ps) {
 }
 
 // ------------------------------------------------------------------ one pass
-bool describe_pass(int prec, int axis, const OpSpec &op, const Layout &lin, const Layout &lout, PassDesc &ps) {
+bool describe_pass(int prec, int axis, const OpSpec &op, const Layout &lin, const Layout &lout, PassDesc &ps,
+                   int batch_threads) {
   PassArgs &a = ps.args;
   memset(&a, 0, sizeof(a));
   ps.axis = axis;
@@ -354,6 +356,7 @@ bool describe_pass(int prec, int axis, const OpSpec &op, const Layout &lin, cons
   a.div_ntile = make_fastdiv((uint32_t)a.ntile_e);
   ps.lds_bytes = per_line * B;
   ps.threads = 256;
+  a.nthreads = 256;
   const unsigned long long nbk = (unsigned long long)a.ntile_e * (unsigned long long)a.nf;
   if (nbk > 0x7fffffffULL) {
     set_error("grid too large");
@@ -369,7 +372,7 @@ bool describe_pass(int prec, int axis, const OpSpec &op, const Layout &lin, cons
   a.dg = (op.cls == CLS_R2C) ? 2 * (nout - 1) : nout;
   a.dmid = a.dg / 2;
   a.scale = 1.0;
-  setup_fast(prec, op, ps);
+  setup_fast(prec, op, ps, batch_threads);
   return true;
 }
 
@@ -387,7 +390,7 @@ static int pass_cost(int axis, const int moi[3], const int moo[3]) {
 static bool same_mo(const int a[3], const int b[3]) { return a[0] == b[0] && a[1] == b[1] && a[2] == b[2]; }
 
 bool plan_chain(int prec, const OpSpec ops[3], const int order[3], const Layout &in, const Layout &out,
-                PassChain &chain) {
+                PassChain &chain, int batch_threads) {
   chain.passes.clear();
   chain.copy_only = false;
   chain.desc.clear();
@@ -413,7 +416,7 @@ bool plan_chain(int prec, const OpSpec ops[3], const int order[3], const Layout 
     }
     PassDesc ps;
     OpSpec cp = {CLS_EMPTY, -1, 0, in.dt, in.dt};
-    if (!describe_pass(prec, stride1_axis(out.mo), cp, in, out, ps)) return false;
+    if (!describe_pass(prec, stride1_axis(out.mo), cp, in, out, ps, batch_threads)) return false;
     chain.passes.push_back(ps);
     chain.desc = "reorder";
     return true;
@@ -476,7 +479,7 @@ bool plan_chain(int prec, const OpSpec ops[3], const int order[3], const Layout 
   char buf[256];
   for (int i = 0; i < k; i++) {
     PassDesc ps;
-    if (!describe_pass(prec, axes[i], ops[axes[i]], lay[i], lay[i + 1], ps)) {
+    if (!describe_pass(prec, axes[i], ops[axes[i]], lay[i], lay[i + 1], ps, batch_threads)) {
       chain.passes.clear();
       return false;
     }
@@ -490,6 +493,72 @@ bool plan_chain(int prec, const OpSpec ops[3], const int order[3], const Layout 
     chain.passes.push_back(ps);
   }
   return true;
+}
+
+// ------------------------------------------------------------------ tuning knobs
+static bool radix_supported(int r) {
+  if (r == 2 || r == 3 || r == 4 || r == 5 || r == 8 || r == 16) return true;
+  if (r < 7 || r > 31 || r % 2 == 0) return false;
+  for (int q = 3; q * q <= r; q += 2)
+    if (r % q == 0) return false;
+  return true;
+}
+
+bool retune_pass(int prec, PassDesc &ps, int B, const int *rad, int nrad) {
+  PassArgs &a = ps.args;
+  if (a.fast || a.cls == CLS_EMPTY || B < 1 || (B & (B - 1)) || nrad < 0 || nrad > P3E_MAX_RADIX_PASSES)
+    return false;
+  long prod = 1;
+  for (int i = 0; i < nrad; i++) {
+    if (!radix_supported(rad[i])) return false;
+    prod *= rad[i];
+  }
+  if (prod != a.L) return false;
+  const size_t lds = 2 * (size_t)a.lpad * 2 * (size_t)prec * B;
+  if (lds > LDS_MAX) return false;
+  for (int i = 0; i < nrad; i++) a.radix[i] = rad[i];
+  a.nrad = nrad;
+  a.B = B;
+  a.logB = 0;
+  while ((1 << a.logB) < B) a.logB++;
+  a.ntile_e = (a.ne + B - 1) / B;
+  a.div_ntile = make_fastdiv((uint32_t)a.ntile_e);
+  ps.lds_bytes = lds;
+  ps.nblocks = (unsigned int)((unsigned long long)a.ntile_e * (unsigned long long)a.nf);
+  return true;
+}
+
+static void factorizations(int n, int maxr, std::vector<int> &cur, std::vector<std::vector<int>> &out) {
+  if (n == 1) {
+    out.push_back(cur);
+    return;
+  }
+  if (cur.size() >= 6) return;
+  static const int R[] = {16, 8, 5, 4, 3, 2, 7, 11, 13, 17, 19, 23, 29, 31};
+  for (int r : R) {
+    if (r > maxr || n % r) continue;
+    cur.push_back(r);
+    factorizations(n / r, r, cur, out);  // non-increasing multisets
+    cur.pop_back();
+  }
+}
+
+std::vector<std::vector<int>> radix_candidates(int W, int cap) {
+  std::vector<std::vector<int>> sets, out;
+  std::vector<int> cur;
+  factorizations(W, 1 << 30, cur, sets);
+  std::sort(sets.begin(), sets.end(),
+            [](const std::vector<int> &x, const std::vector<int> &y) { return x.size() < y.size(); });
+  for (const auto &s : sets) {
+    std::vector<int> desc = s, asc = s;
+    std::sort(desc.begin(), desc.end(), [](int a, int b) { return a > b; });
+    std::sort(asc.begin(), asc.end());
+    for (const auto &c : {desc, asc}) {
+      if (std::find(out.begin(), out.end(), c) == out.end()) out.push_back(c);
+      if ((int)out.size() >= cap) return out;
+    }
+  }
+  return out;
 }
 
 // ------------------------------------------------------------------ buffers

@@ -57,10 +57,18 @@ struct PassChain {
 size_t layout_elems(const Layout &l);
 size_t elem_bytes(int prec, int dt);
 
-bool describe_pass(int prec, int axis, const OpSpec &op, const Layout &lin, const Layout &lout, PassDesc &ps);
+// batch_threads > 0 plans the pass for the batch (step) kernel, whose
+// workgroups have that many threads; 0 plans it for its own launch.
+bool describe_pass(int prec, int axis, const OpSpec &op, const Layout &lin, const Layout &lout, PassDesc &ps,
+                   int batch_threads = 0);
 // ops[a]: operation on logical axis a; order: axes in execution order.
 bool plan_chain(int prec, const OpSpec ops[3], const int order[3], const Layout &in, const Layout &out,
-                PassChain &chain);
+                PassChain &chain, int batch_threads = 0);
+// Re-tile a generic-path pass with B lines per tile and the given radix
+// sequence (the autotuner's knobs).  Returns false if the choice is invalid.
+bool retune_pass(int prec, PassDesc &ps, int B, const int *rad, int nrad);
+// Radix sequences worth trying for a generic-path length W (at most `cap`).
+std::vector<std::vector<int>> radix_candidates(int W, int cap);
 
 enum { BUF_IN = 0, BUF_OUT = 1, BUF_S0 = 2, BUF_S1 = 3 };
 // Destination buffer of each pass (BUF_OUT or scratch), and scratch sizes.

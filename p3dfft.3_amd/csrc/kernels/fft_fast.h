@@ -108,12 +108,20 @@ P3E_HD P3E_INL void fast_tile_coords(const PassArgs &p, uint32_t tile, uint32_t 
 template <typename T, int M>
 P3E_HD void fast_load(const PassArgs &p, uint32_t f, int e0, int nb, cx<T> *buf, int tid, int nth) {
   constexpr int logM = Log2<M>::v;
+  constexpr int E = P3E_FAST_E;
   cx<T> *side = buf + (size_t)p.B * p.lp;
   const long long base = (long long)e0 * p.is_e + (long long)f * p.is_f;
   const int tot = (p.load_order == ORD_LINE ? nb : p.B) * p.nin;
+  // Every thread first issues all its global loads (up to 2E reals or E
+  // complex, tot <= that * nth by construction), then writes them to LDS, so
+  // that many loads are in flight per thread.
   if (p.cls == CLS_R2C) {
     const T *in = reinterpret_cast<const T *>(p.in);
-    for (int i = tid; i < tot; i += nth) {
+    T r[2 * E];
+    int slot[2 * E];
+#pragma unroll
+    for (int q = 0; q < 2 * E; q++) {
+      const int i = tid + q * nth;
       int n, j;
       if (p.load_order == ORD_LINE) {
         j = i >> (logM + 1);
@@ -121,15 +129,48 @@ P3E_HD void fast_load(const PassArgs &p, uint32_t f, int e0, int nb, cx<T> *buf,
       } else {
         j = i & (p.B - 1);
         n = i >> p.logB;
-        if (j >= nb) continue;
       }
-      const T x = in[base + (long long)n * p.is_d + (long long)j * p.is_e];
-      T *slot = reinterpret_cast<T *>(buf + j * p.lp + P3E_PAD(n >> 1));
-      slot[n & 1] = x;
+      slot[q] = -1;
+      if (i < tot && j < nb) {
+        r[q] = in[base + (long long)n * p.is_d + (long long)j * p.is_e];
+        slot[q] = 2 * (j * p.lp + P3E_PAD(n >> 1)) + (n & 1);
+      }
     }
+    T *bufr = reinterpret_cast<T *>(buf);
+#pragma unroll
+    for (int q = 0; q < 2 * E; q++)
+      if (slot[q] >= 0) bufr[slot[q]] = r[q];
   } else {
     const cx<T> *in = reinterpret_cast<const cx<T> *>(p.in);
-    for (int i = tid; i < tot; i += nth) {
+    cx<T> r[E];
+    int slot[E];
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+      const int i = tid + q * nth;
+      int n, j;
+      if (p.load_order == ORD_LINE) {
+        if (p.cls == CLS_C2R) {
+          j = (int)fdiv((uint32_t)i, p.div_nin);
+          n = i - j * p.nin;
+        } else {
+          j = i >> logM;
+          n = i & (M - 1);
+        }
+      } else {
+        j = i & (p.B - 1);
+        n = i >> p.logB;
+      }
+      slot[q] = -1;
+      if (i < tot && j < nb) {
+        r[q] = in[base + (long long)n * p.is_d + (long long)j * p.is_e];
+        slot[q] = (n < M) ? j * p.lp + P3E_PAD(n) : (int)(p.B * p.lp) + j;  // C2R: X_M to the side slot
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < E; q++)
+      if (slot[q] >= 0) buf[slot[q]] = r[q];
+    // C2R has B more elements (the X_M of each line) than E * nth
+    for (int i = tid + E * nth; i < tot; i += nth) {
       int n, j;
       if (p.load_order == ORD_LINE) {
         j = (int)fdiv((uint32_t)i, p.div_nin);
@@ -137,11 +178,11 @@ P3E_HD void fast_load(const PassArgs &p, uint32_t f, int e0, int nb, cx<T> *buf,
       } else {
         j = i & (p.B - 1);
         n = i >> p.logB;
-        if (j >= nb) continue;
       }
+      if (j >= nb) continue;
       const cx<T> x = in[base + (long long)n * p.is_d + (long long)j * p.is_e];
       if (n < M) buf[j * p.lp + P3E_PAD(n)] = x;
-      else side[j] = x;  // C2R: X_M
+      else side[j] = x;
     }
   }
 }
@@ -181,9 +222,16 @@ P3E_HD void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, const cx<T
   const cx<T> *wN = reinterpret_cast<const cx<T> *>(p.wN);
   const long long base = (long long)e0 * p.os_e + (long long)f * p.os_f;
   const int tot = (p.store_order == ORD_LINE ? nb : p.B) * p.nout;
+  constexpr int E = P3E_FAST_E;
   if (p.cls == CLS_C2R) {
     T *out = reinterpret_cast<T *>(p.out);
-    for (int i = tid; i < tot; i += nth) {
+    const T *bufr = reinterpret_cast<const T *>(buf);
+    // 2E reals per thread: LDS reads first, then the global stores
+    T y[2 * E];
+    long long off[2 * E];
+#pragma unroll
+    for (int q = 0; q < 2 * E; q++) {
+      const int i = tid + q * nth;
       int k, j;
       if (p.store_order == ORD_LINE) {
         j = i >> (logM + 1);
@@ -191,20 +239,29 @@ P3E_HD void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, const cx<T
       } else {
         j = i & (p.B - 1);
         k = i >> p.logB;
-        if (j >= nb) continue;
       }
-      const T *slot = reinterpret_cast<const T *>(buf + j * p.lp + P3E_PAD(k >> 1));
-      T y = slot[k & 1];
-      if (p.has_scale) y *= (T)p.scale;
-      out[base + (long long)k * p.os_d + (long long)j * p.os_e] = y;
+      off[q] = -1;
+      if (i < tot && j < nb) {
+        y[q] = bufr[2 * (j * p.lp + P3E_PAD(k >> 1)) + (k & 1)];
+        off[q] = base + (long long)k * p.os_d + (long long)j * p.os_e;
+      }
     }
+#pragma unroll
+    for (int q = 0; q < 2 * E; q++)
+      if (off[q] >= 0) out[off[q]] = p.has_scale ? y[q] * (T)p.scale : y[q];
   } else {
     cx<T> *out = reinterpret_cast<cx<T> *>(p.out);
+#pragma unroll 4
     for (int i = tid; i < tot; i += nth) {
       int k, j;
       if (p.store_order == ORD_LINE) {
-        j = (int)fdiv((uint32_t)i, p.div_nout);
-        k = i - j * p.nout;
+        if (p.cls == CLS_R2C) {
+          j = (int)fdiv((uint32_t)i, p.div_nout);
+          k = i - j * p.nout;
+        } else {
+          j = i >> logM;
+          k = i & (M - 1);
+        }
       } else {
         j = i & (p.B - 1);
         k = i >> p.logB;
@@ -234,22 +291,27 @@ P3E_HD void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, const cx<T
   }
 }
 
-// Whole tile, given a pass runner (device: this thread's halves with
-// barriers; host tests: every thread's read half, then every write half) and
-// a barrier for the phases that are plain loops over the tile.
-template <typename T, int M, typename Run, typename Bar>
-P3E_HD void fast_tile(const PassArgs &p, uint32_t tile, cx<T> *buf, int tid, int nth, Run &run, Bar bar) {
+// Whole tile.  `each(f)` runs phase f for the calling thread and ends with a
+// barrier (device), or runs it for every thread of the workgroup in turn
+// (host build); `run` does the same for the two halves of a radix pass.
+// Only threads tid < nth work (a batch workgroup may be larger than the
+// pass needs); the others take part in the barriers.
+template <typename T, int M, typename Run, typename Each>
+P3E_HD void fast_tile(const PassArgs &p, uint32_t tile, cx<T> *buf, int nth, Run &run, Each &each) {
   uint32_t f;
   int e0, nb;
   fast_tile_coords(p, tile, &f, &e0, &nb);
-  fast_load<T, M>(p, f, e0, nb, buf, tid, nth);
-  bar();
-  if (p.cls == CLS_C2R) {
-    fast_c2r_merge<T, M>(p, nb, buf, tid, nth);
-    bar();
-  }
+  each([&](int tid) {
+    if (tid < nth) fast_load<T, M>(p, f, e0, nb, buf, tid, nth);
+  });
+  if (p.cls == CLS_C2R)
+    each([&](int tid) {
+      if (tid < nth) fast_c2r_merge<T, M>(p, nb, buf, tid, nth);
+    });
   fft_pow2<T, M>(run, buf, p.lp, nb, (p.cls == CLS_R2C) ? -1 : p.sign, reinterpret_cast<const cx<T> *>(p.tw));
-  fast_store<T, M>(p, f, e0, nb, buf, tid, nth);
+  each([&](int tid) {
+    if (tid < nth) fast_store<T, M>(p, f, e0, nb, buf, tid, nth);
+  });
 }
 
 }  // namespace p3e

@@ -36,9 +36,18 @@ template <typename T> struct DevPassRun {
   int tid, nth;
   template <class PS> __device__ void operator()(PS &ps) const {
     cx<T> v[P3E_FAST_E];
-    ps.read(tid, nth, v);
+    if (tid < nth) ps.read(tid, nth, v);
     __syncthreads();
-    ps.write(tid, nth, v);
+    if (tid < nth) ps.write(tid, nth, v);
+    __syncthreads();
+  }
+};
+
+// A plain phase of the power-of-two path for this thread, then a barrier.
+struct DevEach {
+  int tid;
+  template <class F> __device__ void operator()(F f) const {
+    f(tid);
     __syncthreads();
   }
 };
@@ -49,7 +58,55 @@ __global__ void __launch_bounds__(P3E_FAST_MAX_THREADS) pass_kernel_pow2(PassArg
   extern __shared__ unsigned char smem_raw[];
   cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
   DevPassRun<T> run{(int)threadIdx.x, (int)blockDim.x};
-  fast_tile<T, M>(p, blockIdx.x, buf, threadIdx.x, blockDim.x, run, DevBarrier());
+  DevEach each{(int)threadIdx.x};
+  fast_tile<T, M>(p, blockIdx.x, buf, blockDim.x, run, each);
+}
+
+// Batch step: a grid-stride loop over a work list of (pass, tile) entries
+// drawn from many transforms (and from different stages of each), every
+// workgroup with P3E_STEP_THREADS threads and LDS for the largest tile.
+// Stages of one transform are ordered by launch boundaries: on MI355X each
+// XCD has its own L2, so a kernel boundary is the cheap way to make one
+// stage's output visible to the next stage's tiles on other XCDs.
+template <typename T, int M>
+__device__ void step_pow2(const PassArgs &p, uint32_t tile, cx<T> *buf) {
+  DevPassRun<T> run{(int)threadIdx.x, p.nthreads};
+  DevEach each{(int)threadIdx.x};
+  fast_tile<T, M>(p, tile, buf, p.nthreads, run, each);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(P3E_STEP_THREADS)
+    step_kernel(const PassArgs *__restrict__ table, const WorkItem *__restrict__ work, int nwork) {
+  extern __shared__ unsigned char smem_raw[];
+  __shared__ PassArgs sp;
+  cx<T> *bufA = reinterpret_cast<cx<T> *>(smem_raw);
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const WorkItem wi = work[w];
+    __syncthreads();  // the previous tile is done with sp and the LDS
+    {
+      const uint32_t *src = reinterpret_cast<const uint32_t *>(table + wi.pass);
+      uint32_t *dst = reinterpret_cast<uint32_t *>(&sp);
+      for (int i = threadIdx.x; i < (int)(sizeof(PassArgs) / 4); i += blockDim.x) dst[i] = src[i];
+    }
+    __syncthreads();
+    if (sp.fast) {
+      switch (sp.M) {
+        case 16: step_pow2<T, 16>(sp, wi.tile, bufA); break;
+        case 32: step_pow2<T, 32>(sp, wi.tile, bufA); break;
+        case 64: step_pow2<T, 64>(sp, wi.tile, bufA); break;
+        case 128: step_pow2<T, 128>(sp, wi.tile, bufA); break;
+        case 256: step_pow2<T, 256>(sp, wi.tile, bufA); break;
+        case 512: step_pow2<T, 512>(sp, wi.tile, bufA); break;
+        case 1024: step_pow2<T, 1024>(sp, wi.tile, bufA); break;
+        case 2048: step_pow2<T, 2048>(sp, wi.tile, bufA); break;
+        default: step_pow2<T, 4096>(sp, wi.tile, bufA); break;
+      }
+    } else {
+      cx<T> *bufB = bufA + (size_t)sp.B * sp.lpad;
+      run_tile<T>(sp, wi.tile, bufA, bufB, threadIdx.x, blockDim.x, DevBarrier());
+    }
+  }
 }
 
 template <typename T, int M>
@@ -120,6 +177,38 @@ extern "C" int p3e_launch_pass(const p3e::PassArgs *a, int prec, size_t lds_byte
     hipLaunchKernelGGL(p3e::pass_kernel<float>, dim3(nblocks), dim3(threads), lds_bytes, st, *a);
   }
   return (int)hipGetLastError();
+}
+
+extern "C" int p3e_launch_step(const void *table, const void *work, int nwork, int prec, size_t lds,
+                               unsigned int grid, hipStream_t st) {
+  if (nwork <= 0) return 0;
+  const void *fn = prec == 8 ? (const void *)p3e::step_kernel<double> : (const void *)p3e::step_kernel<float>;
+  if (lds > 65536) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (prec == 8)
+    hipLaunchKernelGGL(p3e::step_kernel<double>, dim3(grid), dim3(P3E_STEP_THREADS), lds, st,
+                       (const p3e::PassArgs *)table, (const p3e::WorkItem *)work, nwork);
+  else
+    hipLaunchKernelGGL(p3e::step_kernel<float>, dim3(grid), dim3(P3E_STEP_THREADS), lds, st,
+                       (const p3e::PassArgs *)table, (const p3e::WorkItem *)work, nwork);
+  return (int)hipGetLastError();
+}
+
+// Workgroups of the step kernel resident per CU for a given dynamic LDS size.
+extern "C" int p3e_step_occupancy(int prec, size_t lds) {
+  static size_t last_lds[2] = {(size_t)-1, (size_t)-1};
+  static int last_n[2] = {1, 1};
+  const int k = prec == 8 ? 1 : 0;
+  if (last_lds[k] == lds) return last_n[k];
+  const void *fn = prec == 8 ? (const void *)p3e::step_kernel<double> : (const void *)p3e::step_kernel<float>;
+  if (lds > 65536) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, P3E_STEP_THREADS, lds) != hipSuccess || n < 1) {
+    (void)hipGetLastError();
+    n = 1;
+  }
+  last_lds[k] = lds;
+  last_n[k] = n;
+  return n;
 }
 
 extern "C" int p3e_launch_deriv(const void *in, void *out, long long n, int s0, int s1, int ldir, int g,

@@ -73,7 +73,16 @@ struct PassArgs {
   int M, logM;       // complex FFT length of the fast path (N, or N/2 for R2C/C2R)
   int lp;            // LDS pitch of one line in the fast path (padded)
   const void *wN;    // R2C/C2R: M+1 entries exp(-2 pi i k / N)
+  int nthreads;      // threads that work on a tile (a batch launch may have more)
 };
+
+// One entry of a batch work list: tile `tile` of the pass table[pass].
+struct WorkItem {
+  uint32_t pass, tile;
+};
+
+// Workgroup size of the batch (step) kernel.
+#define P3E_STEP_THREADS 256
 
 // elements held in registers per thread by the fast path
 #define P3E_FAST_E 16

@@ -38,12 +38,21 @@ template <typename T> struct HostPassRun {
   }
 };
 
+// A plain phase: every thread of the workgroup in turn.
+struct HostEach {
+  int nth;
+  template <class F> void operator()(F f) const {
+    for (int tid = 0; tid < nth; tid++) f(tid);
+  }
+};
+
 template <typename T, int M> static int run_pow2(const PassDesc &pd, const PassArgs &a) {
   HostPassRun<T> run{pd.threads, std::vector<cx<T>>((size_t)pd.threads * P3E_FAST_E)};
+  HostEach each{pd.threads};
   std::vector<cx<T>> buf((size_t)a.B * a.lp + a.B);
   for (unsigned int t = 0; t < pd.nblocks; t++) {
     for (auto &x : buf) x = cx<T>{(T)1e30, (T)-1e30};
-    fast_tile<T, M>(a, t, buf.data(), 0, 1, run, HostBarrier());
+    fast_tile<T, M>(a, t, buf.data(), pd.threads, run, each);
   }
   return 0;
 }
@@ -92,7 +101,8 @@ extern "C" {
 // Returns 0, or -1 with the planner's message in emu_error().
 int emu_exec(int prec, const int *cls, const int *sign, const int *kind, const int *idt, const int *odt,
              const int *order, const int *in_dims, const int *in_mo, int in_dt, const int *out_dims,
-             const int *out_mo, int out_dt, const void *in, void *out, int idir, char *desc, int desc_len) {
+             const int *out_mo, int out_dt, const void *in, void *out, int idir, char *desc, int desc_len,
+             int batch_threads) {
   OpSpec ops[3];
   for (int a = 0; a < 3; a++) ops[a] = OpSpec{cls[a], sign[a], kind[a], idt[a], odt[a]};
   Layout li, lo;
@@ -105,7 +115,7 @@ int emu_exec(int prec, const int *cls, const int *sign, const int *kind, const i
   li.dt = in_dt;
   lo.dt = out_dt;
   PassChain ch;
-  if (!plan_chain(prec, ops, order, li, lo, ch)) return -1;
+  if (!plan_chain(prec, ops, order, li, lo, ch, batch_threads)) return -1;
   if (desc && desc_len > 0) {
     strncpy(desc, ch.desc.c_str(), desc_len - 1);
     desc[desc_len - 1] = 0;

@@ -40,7 +40,7 @@ def ops_for(types):
     return cls, sign, kind, idt, odt
 
 
-def exec3d(types, in_dims, mo1, out_dims, mo2, data, prec, idir=-1, order=None):
+def exec3d(types, in_dims, mo1, out_dims, mo2, data, prec, idir=-1, order=None, batch_threads=0):
     """Run the transform with the product's planner and kernel code on the CPU.
     data: storage-order array; returns the storage-order output."""
     cls, sign, kind, idt, odt = ops_for(types)
@@ -57,7 +57,7 @@ def exec3d(types, in_dims, mo1, out_dims, mo2, data, prec, idir=-1, order=None):
     rc = lib().emu_exec(prec, I3(*cls), I3(*sign), I3(*kind), I3(*idt), I3(*odt), I3(*order),
                         I3(*in_dims), I3(*mo1), in_dt, I3(*out_dims), I3(*mo2), out_dt,
                         data.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p),
-                        int(idir), desc, 512)
+                        int(idir), desc, 512, int(batch_threads))
     if rc != 0:
         raise RuntimeError(lib().emu_error().decode())
     return out, desc.value.decode()
@@ -79,7 +79,7 @@ def exec1d(type_id, d, in_dims, mo1, out_dims, mo2, data, prec, deriv=False):
     rc = lib().emu_exec(prec, I3(*cls), I3(*sign), I3(*kind), I3(*idt), I3(*odt), I3(*order),
                         I3(*in_dims), I3(*mo1), ti.dt1, I3(*out_dims), I3(*mo2), ti.dt2,
                         data.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p),
-                        d if deriv else -1, desc, 512)
+                        d if deriv else -1, desc, 512, 0)
     if rc != 0:
         raise RuntimeError(lib().emu_error().decode())
     return out

@@ -149,3 +149,17 @@ def test_r2c_c2r_roundtrip_fp32_fp64(prec):
     spec, _ = emu.exec3d(tf, dims, mo1, [17, 16, 8], mo2, x, prec)
     back, _ = emu.exec3d(tb, [17, 16, 8], mo2, dims, mo1, spec, prec)
     assert C.rel_l2(back / np.prod(dims), x) < (1e-13 if prec == 8 else 1e-6)
+
+
+@pytest.mark.parametrize("dims", [[32, 48, 64], [96, 120, 128], [128, 32, 96], [48, 48, 120]])
+def test_batch_mode_plans(dims):
+    """Passes planned for the batch (step) kernel: 256-thread workgroups,
+    power-of-two tiles of up to 256 lines, for the batched-transform sizes."""
+    x = (np.random.default_rng(sum(dims)).standard_normal(orc.storage_shape(dims, [0, 1, 2])) +
+         1j * np.random.default_rng(1).standard_normal(orc.storage_shape(dims, [0, 1, 2])))
+    for prec in (8, 4):
+        t = [C.CF_D] * 3 if prec == 8 else [C.CF_S] * 3
+        xi = x if prec == 8 else x.astype(np.complex64)
+        got, desc = emu.exec3d(t, dims, [0, 1, 2], dims, [0, 1, 2], xi, prec, batch_threads=256)
+        ref = orc.exec3d([C.CF_D] * 3, dims, [0, 1, 2], [0, 1, 2], xi.astype(np.complex128))
+        assert C.rel_l2(got, ref) < TOL[prec], desc
