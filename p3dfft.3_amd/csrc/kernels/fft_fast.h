@@ -112,16 +112,21 @@ P3E_HD void fast_load(const PassArgs &p, uint32_t f, int e0, int nb, cx<T> *buf,
   cx<T> *side = buf + (size_t)p.B * p.lp;
   const long long base = (long long)e0 * p.is_e + (long long)f * p.is_f;
   const int tot = (p.load_order == ORD_LINE ? nb : p.B) * p.nin;
-  // Every thread first issues all its global loads (up to 2E reals or E
-  // complex, tot <= that * nth by construction), then writes them to LDS, so
-  // that many loads are in flight per thread.
+  // Every thread owns up to 2E reals or E complex of the tile (tot <= that
+  // * nth by construction).  It issues CH complex (2CH real) global loads at
+  // a time before writing them to LDS, so that many loads are in flight.
+  // The staging arrays are arrays of scalars: arrays of cx<double> are not
+  // split into registers by the compiler and would live in scratch.
+  constexpr int CH = E;
   if (p.cls == CLS_R2C) {
     const T *in = reinterpret_cast<const T *>(p.in);
-    T r[2 * E];
-    int slot[2 * E];
 #pragma unroll
-    for (int q = 0; q < 2 * E; q++) {
-      const int i = tid + q * nth;
+    for (int c = 0; c < 2 * E; c += 2 * CH) {
+    T r[2 * CH];
+    int slot[2 * CH];
+#pragma unroll
+    for (int q = 0; q < 2 * CH; q++) {
+      const int i = tid + (c + q) * nth;
       int n, j;
       if (p.load_order == ORD_LINE) {
         j = i >> (logM + 1);
@@ -130,23 +135,26 @@ P3E_HD void fast_load(const PassArgs &p, uint32_t f, int e0, int nb, cx<T> *buf,
         j = i & (p.B - 1);
         n = i >> p.logB;
       }
-      slot[q] = -1;
-      if (i < tot && j < nb) {
-        r[q] = in[base + (long long)n * p.is_d + (long long)j * p.is_e];
-        slot[q] = 2 * (j * p.lp + P3E_PAD(n >> 1)) + (n & 1);
-      }
+      // out-of-tile entries load the tile's first element (always valid)
+      // and are not written: unconditional loads keep r[] in registers
+      const bool ok = i < tot && j < nb;
+      r[q] = in[ok ? base + (long long)n * p.is_d + (long long)j * p.is_e : base];
+      slot[q] = ok ? 2 * (j * p.lp + P3E_PAD(n >> 1)) + (n & 1) : -1;
     }
     T *bufr = reinterpret_cast<T *>(buf);
 #pragma unroll
-    for (int q = 0; q < 2 * E; q++)
+    for (int q = 0; q < 2 * CH; q++)
       if (slot[q] >= 0) bufr[slot[q]] = r[q];
+    }
   } else {
     const cx<T> *in = reinterpret_cast<const cx<T> *>(p.in);
-    cx<T> r[E];
-    int slot[E];
 #pragma unroll
-    for (int q = 0; q < E; q++) {
-      const int i = tid + q * nth;
+    for (int c = 0; c < E; c += CH) {
+    T rx[CH], ry[CH];
+    int slot[CH];
+#pragma unroll
+    for (int q = 0; q < CH; q++) {
+      const int i = tid + (c + q) * nth;
       int n, j;
       if (p.load_order == ORD_LINE) {
         if (p.cls == CLS_C2R) {
@@ -160,15 +168,17 @@ P3E_HD void fast_load(const PassArgs &p, uint32_t f, int e0, int nb, cx<T> *buf,
         j = i & (p.B - 1);
         n = i >> p.logB;
       }
-      slot[q] = -1;
-      if (i < tot && j < nb) {
-        r[q] = in[base + (long long)n * p.is_d + (long long)j * p.is_e];
-        slot[q] = (n < M) ? j * p.lp + P3E_PAD(n) : (int)(p.B * p.lp) + j;  // C2R: X_M to the side slot
-      }
+      const bool ok = i < tot && j < nb;
+      const cx<T> v = in[ok ? base + (long long)n * p.is_d + (long long)j * p.is_e : base];
+      rx[q] = v.x;
+      ry[q] = v.y;
+      // C2R: X_M goes to the side slot
+      slot[q] = !ok ? -1 : (n < M) ? j * p.lp + P3E_PAD(n) : (int)(p.B * p.lp) + j;
     }
 #pragma unroll
-    for (int q = 0; q < E; q++)
-      if (slot[q] >= 0) buf[slot[q]] = r[q];
+    for (int q = 0; q < CH; q++)
+      if (slot[q] >= 0) buf[slot[q]] = mk(rx[q], ry[q]);
+    }
     // C2R has B more elements (the X_M of each line) than E * nth
     for (int i = tid + E * nth; i < tot; i += nth) {
       int n, j;
@@ -226,12 +236,16 @@ P3E_HD void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, const cx<T
   if (p.cls == CLS_C2R) {
     T *out = reinterpret_cast<T *>(p.out);
     const T *bufr = reinterpret_cast<const T *>(buf);
-    // 2E reals per thread: LDS reads first, then the global stores
-    T y[2 * E];
-    long long off[2 * E];
+    // 2E reals per thread, 2CH at a time: LDS reads first, then the global
+    // stores
+    constexpr int CH = E;
 #pragma unroll
-    for (int q = 0; q < 2 * E; q++) {
-      const int i = tid + q * nth;
+    for (int c = 0; c < 2 * E; c += 2 * CH) {
+    T y[2 * CH];
+    long long off[2 * CH];
+#pragma unroll
+    for (int q = 0; q < 2 * CH; q++) {
+      const int i = tid + (c + q) * nth;
       int k, j;
       if (p.store_order == ORD_LINE) {
         j = i >> (logM + 1);
@@ -240,15 +254,14 @@ P3E_HD void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, const cx<T
         j = i & (p.B - 1);
         k = i >> p.logB;
       }
-      off[q] = -1;
-      if (i < tot && j < nb) {
-        y[q] = bufr[2 * (j * p.lp + P3E_PAD(k >> 1)) + (k & 1)];
-        off[q] = base + (long long)k * p.os_d + (long long)j * p.os_e;
-      }
+      const bool ok = i < tot && j < nb;
+      y[q] = bufr[ok ? 2 * (j * p.lp + P3E_PAD(k >> 1)) + (k & 1) : 0];
+      off[q] = ok ? base + (long long)k * p.os_d + (long long)j * p.os_e : -1;
     }
 #pragma unroll
-    for (int q = 0; q < 2 * E; q++)
+    for (int q = 0; q < 2 * CH; q++)
       if (off[q] >= 0) out[off[q]] = p.has_scale ? y[q] * (T)p.scale : y[q];
+    }
   } else {
     cx<T> *out = reinterpret_cast<cx<T> *>(p.out);
 #pragma unroll 4

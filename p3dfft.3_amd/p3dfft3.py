@@ -211,3 +211,54 @@ def set_fast_path(enable=True):
     """Plans made afterwards use the power-of-two kernel where it applies
     (default) or always the generic kernel."""
     lib().p3dfft_hip_set_fast_path(int(bool(enable)))
+
+
+# ------------------------------------------------------------ batches
+
+def batch_plan(dims_in, dims_out, types, mo_in=(0, 1, 2), mo_out=(0, 1, 2), tune=False):
+    """Batch of independent 3D transforms (include/p3dfft_hip.h): dims_in and
+    dims_out are lists of per-item logical dims; types are three 1D type IDs."""
+    L = lib()
+    n = len(dims_in)
+    if len(dims_out) != n:
+        raise ValueError("dims_in and dims_out differ in length")
+    IN = ctypes.c_int * (3 * n)
+    di = IN(*[int(v) for d in dims_in for v in d])
+    do = IN(*[int(v) for d in dims_out for v in d])
+    L.p3dfft_hip_batch_plan.argtypes = [ctypes.c_int, IN, IN, I3, I3, I3, ctypes.c_int]
+    h = L.p3dfft_hip_batch_plan(n, di, do, I3(*types), I3(*mo_in), I3(*mo_out), int(bool(tune)))
+    if h < 0:
+        raise RuntimeError(last_error())
+    return h
+
+
+def batch_exec(h, ins, outs, stream=None, sync=True):
+    """ins/outs: per-item device addresses."""
+    L = lib()
+    n = len(ins)
+    A = VP * n
+    L.p3dfft_hip_batch_exec.argtypes = [ctypes.c_int, A, A, VP, ctypes.c_int]
+    rc = L.p3dfft_hip_batch_exec(h, A(*ins), A(*outs), stream, int(bool(sync)))
+    if rc != 0:
+        raise RuntimeError(last_error())
+
+
+def batch_set_graph(h, enable=True):
+    L = lib()
+    L.p3dfft_hip_batch_set_graph.argtypes = [ctypes.c_int, ctypes.c_int]
+    if L.p3dfft_hip_batch_set_graph(h, int(bool(enable))) != 0:
+        raise RuntimeError(last_error())
+
+
+def batch_describe(h):
+    L = lib()
+    L.p3dfft_hip_batch_describe.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+    buf = ctypes.create_string_buffer(1 << 16)
+    L.p3dfft_hip_batch_describe(h, buf, len(buf))
+    return buf.value.decode()
+
+
+def batch_destroy(h):
+    L = lib()
+    L.p3dfft_hip_batch_destroy.argtypes = [ctypes.c_int]
+    return L.p3dfft_hip_batch_destroy(h)
