@@ -71,6 +71,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="after the timed steps, run a plain 256 MiB device copy three times (the "
+                         "known-size kernel that scales HBM byte counters under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs the 1x1 (single-GPU) transform path only; "
@@ -138,6 +141,13 @@ def main():
     achieved = per_launch_bytes / (avg_launch_ms * 1e-3) / 1e9
     log(f"step {t_step * 1e3:.3f} ms  {value:.1f} GB/s  roundtrip rel L2 {err:.2e}")
     log("forward pass avg ms:", [round(v, 4) for v in avg_f], "backward:", [round(v, 4) for v in avg_b])
+    if args.copy_probe:
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy probe: 3 x 268435456-byte device copies done")
     traffic = load_traffic()
     cpu = None
     if not args.no_cpu_baseline:
@@ -162,7 +172,7 @@ def main():
                                "sin(x)sin(2y)sin(3z)+N(0,1e-3)",
                    "grid": [n, n, n], "types": "R2CFFT_S,CFFT_FORWARD_S,CFFT_FORWARD_S / back",
                    "b_alg_bytes": balg, "parallelism": "1 GPU, 1x1 processor grid"},
-        "roofline": {"kernel": "p3e::pass_kernel<float>", "bound": "hbm", "achieved": round(achieved, 1),
+        "roofline": {"kernel": "p3e::pass_kernel_pow2<float,M> (M=512 C2C, M=256 R2C/C2R; all 6 stage launches)", "bound": "hbm", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "bytes_per_launch_alg": per_launch_bytes,
                      "avg_launch_ms": round(avg_launch_ms, 5)},

@@ -173,6 +173,7 @@ static float time_candidate(int prec, PassDesc &pd, void *din, void *dout, PassA
 static void tune_pass(int prec, PassDesc &pd, hipStream_t st) {
   PassArgs &a = pd.args;
   if (a.fast || a.cls == CLS_EMPTY) return;
+  if (!tables_ready(pd)) return;  // never launch a pass whose tables are not on the device
   load_cache();
   const std::string key = tune_key(prec, pd);
   auto it = g_tune.find(key);
@@ -261,11 +262,19 @@ BatchPlan *batch_create(int prec, int nitems, const OpSpec ops[3], const int ord
     }
   }
   b->nstages = (int)b->chains[0].passes.size();
-  // constant tables, tuning
+  // constant tables first (the tuner launches the pass, which reads them;
+  // re-tiling keeps the work length and so the tables), then tuning
   for (int i = 0; i < nitems; i++)
     for (PassDesc &pd : b->chains[i].passes) {
+      const bool up = upload_pass_tables(prec, pd, b->owned);
+      if (!up || !tables_ready(pd)) {
+        if (up) set_error("batch: a constant table has no device copy");
+        batch_destroy(b);
+        return nullptr;
+      }
       if (tune) tune_pass(prec, pd, st);
-      if (!upload_pass_tables(prec, pd, b->owned)) {
+      if (!tables_ready(pd)) {
+        set_error("batch: tuning lost a constant table");
         batch_destroy(b);
         return nullptr;
       }

@@ -111,7 +111,9 @@ Plan *plan_create(int prec, const OpSpec ops[3], const int order[3], const Layou
   for (PassDesc &d : chain.passes) {
     Pass ps;
     ps.d = d;
-    if (!upload_tables(prec, ps)) {
+    const bool up = upload_tables(prec, ps);
+    if (!up || !tables_ready(ps.d)) {
+      if (up) set_error("pass %zu: a constant table has no device copy", p->passes.size());
       free_tables(ps);
       plan_destroy(p);
       return nullptr;

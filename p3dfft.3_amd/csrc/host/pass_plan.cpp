@@ -528,6 +528,15 @@ bool retune_pass(int prec, PassDesc &ps, int B, const int *rad, int nrad) {
   return true;
 }
 
+bool tables_ready(const PassDesc &ps) {
+  const PassArgs &a = ps.args;
+  const std::vector<cld> *h[6] = {&ps.tw, &ps.chirp, &ps.chirp_hat, &ps.pre, &ps.post, &ps.wN};
+  const void *d[6] = {a.tw, a.chirp, a.chirp_hat, a.pre, a.post, a.wN};
+  for (int i = 0; i < 6; i++)
+    if (!h[i]->empty() && !d[i]) return false;
+  return true;
+}
+
 static void factorizations(int n, int maxr, std::vector<int> &cur, std::vector<std::vector<int>> &out) {
   if (n == 1) {
     out.push_back(cur);

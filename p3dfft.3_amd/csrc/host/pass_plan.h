@@ -67,6 +67,11 @@ bool plan_chain(int prec, const OpSpec ops[3], const int order[3], const Layout 
 // Re-tile a generic-path pass with B lines per tile and the given radix
 // sequence (the autotuner's knobs).  Returns false if the choice is invalid.
 bool retune_pass(int prec, PassDesc &ps, int B, const int *rad, int nrad);
+// True when every constant table the planner built for this pass (twiddles,
+// chirps, pre/post factors, R2C/C2R split factors) has a device address in
+// its args: the kernels read those tables whenever the planner built them.
+// Checked before every launch; a pass that fails it is never launched.
+bool tables_ready(const PassDesc &ps);
 // Radix sequences worth trying for a generic-path length W (at most `cap`).
 std::vector<std::vector<int>> radix_candidates(int W, int cap);
 

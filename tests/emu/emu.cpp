@@ -176,4 +176,39 @@ int emu_assign(int prec, const int *cls, const int *sign, const int *kind, const
   return (int)ch.passes.size();
 }
 
+// The launch guard: number of passes whose tables are not ready with the
+// planner's null table addresses (*before) and with every built table given
+// an address (*after).
+int emu_tables_ready(int prec, const int *cls, const int *sign, const int *kind, const int *idt, const int *odt,
+                     const int *order, const int *in_dims, const int *in_mo, int in_dt, const int *out_dims,
+                     const int *out_mo, int out_dt, int *before, int *after) {
+  OpSpec ops[3];
+  for (int a = 0; a < 3; a++) ops[a] = OpSpec{cls[a], sign[a], kind[a], idt[a], odt[a]};
+  Layout li, lo;
+  for (int i = 0; i < 3; i++) {
+    li.dims[i] = in_dims[i];
+    li.mo[i] = in_mo[i];
+    lo.dims[i] = out_dims[i];
+    lo.mo[i] = out_mo[i];
+  }
+  li.dt = in_dt;
+  lo.dt = out_dt;
+  PassChain ch;
+  if (!plan_chain(prec, ops, order, li, lo, ch)) return -1;
+  *before = *after = 0;
+  static const char dummy[16] = {0};
+  for (PassDesc &pd : ch.passes) {
+    if (!tables_ready(pd)) ++*before;
+    PassArgs &a = pd.args;
+    if (!pd.tw.empty()) a.tw = dummy;
+    if (!pd.chirp.empty()) a.chirp = dummy;
+    if (!pd.chirp_hat.empty()) a.chirp_hat = dummy;
+    if (!pd.pre.empty()) a.pre = dummy;
+    if (!pd.post.empty()) a.post = dummy;
+    if (!pd.wN.empty()) a.wN = dummy;
+    if (!tables_ready(pd)) ++*after;
+  }
+  return (int)ch.passes.size();
+}
+
 }  // extern "C"

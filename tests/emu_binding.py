@@ -83,3 +83,18 @@ def exec1d(type_id, d, in_dims, mo1, out_dims, mo2, data, prec, deriv=False):
     if rc != 0:
         raise RuntimeError(lib().emu_error().decode())
     return out
+
+
+def tables_ready_counts(types, in_dims, mo1, out_dims, mo2, prec):
+    """(passes, not ready with null table addresses, not ready with all set)."""
+    cls, sign, kind, idt, odt = ops_for(types)
+    order = orc.find_order(types, mo1, mo2)
+    in_dt = orc.type_info(types[order[0]]).dt1
+    out_dt = orc.type_info(types[order[2]]).dt2
+    b, a = ctypes.c_int(), ctypes.c_int()
+    n = lib().emu_tables_ready(prec, I3(*cls), I3(*sign), I3(*kind), I3(*idt), I3(*odt), I3(*order),
+                               I3(*in_dims), I3(*mo1), in_dt, I3(*out_dims), I3(*mo2), out_dt,
+                               ctypes.byref(b), ctypes.byref(a))
+    if n < 0:
+        raise RuntimeError(lib().emu_error().decode())
+    return n, b.value, a.value

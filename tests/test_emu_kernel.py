@@ -163,3 +163,17 @@ def test_batch_mode_plans(dims):
         got, desc = emu.exec3d(t, dims, [0, 1, 2], dims, [0, 1, 2], xi, prec, batch_threads=256)
         ref = orc.exec3d([C.CF_D] * 3, dims, [0, 1, 2], [0, 1, 2], xi.astype(np.complex128))
         assert C.rel_l2(got, ref) < TOL[prec], desc
+
+
+@pytest.mark.parametrize("types,dims", [
+    ([C.CF_S] * 3, [48, 96, 120]),          # generic path: twiddles
+    ([C.R2C_D, C.CF_D, C.CF_D], [64, 32, 16]),  # power-of-two path: twiddles + split factors
+    ([C.CF_D] * 3, [139, 8, 4]),            # Bluestein: chirp tables
+    ([C.r2r_id("DCT2", False, 8)] * 3, [12, 10, 9]),  # post-twiddles
+])
+def test_launch_guard_requires_device_tables(types, dims):
+    """Every transform pass has tables, so with the planner's null table
+    addresses none may be launched; with addresses for all, all may."""
+    od = C.out_dims(types, dims)
+    n, before, after = emu.tables_ready_counts(types, dims, [0, 1, 2], od, [0, 1, 2], orc.type_info(types[0]).prec)
+    assert n == 3 and before == 3 and after == 0
