@@ -155,7 +155,12 @@ static void setup_fast(int prec, const OpSpec &op, PassDesc &ps, int batch_threa
   const int nrad = fast_radices(M, rad);
   if (nrad == 0) return;
   const size_t esz = 2 * (size_t)prec;
-  const int lp = M + M / 16;
+  // Line pitch in LDS: room for the padded line (P3E_PAD(M-1) < M + M/16),
+  // rounded up to 1 mod 32 complex elements.  The load and store phases of
+  // tile order touch one element of each of many lines at once; a pitch of
+  // 1 mod 32 puts consecutive lines on consecutive LDS bank pairs (FP32) or
+  // quads (FP64) instead of on the same banks.
+  const int lp = ((M + M / 16 + 31) & ~31) + 1;
   const long maxth = batch_threads > 0 ? batch_threads : P3E_FAST_MAX_THREADS;
   int B = 1;
   const int Bmax = std::min(batch_threads > 0 ? 256 : 64, next_pow2(a.ne));
