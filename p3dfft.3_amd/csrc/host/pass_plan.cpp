@@ -336,6 +336,8 @@ bool describe_pass(int prec, int axis, const OpSpec &op, const Layout &lin, cons
   if (is[d] != 1) e = (is[o1] <= is[o2]) ? o1 : o2;
   else e = (os[o1] <= os[o2]) ? o1 : o2;
   const int f = 3 - d - e;
+  ps.axis_e = e;
+  ps.axis_f = f;
   a.is_d = is[d]; a.is_e = is[e]; a.is_f = is[f];
   a.os_d = os[d]; a.os_e = os[e]; a.os_f = os[f];
   a.ne = lin.dims[e];
@@ -530,6 +532,48 @@ bool retune_pass(int prec, PassDesc &ps, int B, const int *rad, int nrad) {
   a.div_ntile = make_fastdiv((uint32_t)a.ntile_e);
   ps.lds_bytes = lds;
   ps.nblocks = (unsigned int)((unsigned long long)a.ntile_e * (unsigned long long)a.nf);
+  return true;
+}
+
+bool plan_poisson(int prec, const int dims[3], const int mo_real[3], const int mo_spec[3], PassChain &fwd,
+                  PassChain &bwd, int batch_threads) {
+  for (int a = 0; a < 3; a++)
+    if (dims[a] < 2) {
+      set_error("Poisson grid needs at least 2 points per axis");
+      return false;
+    }
+  const OpSpec f_ops[3] = {{CLS_R2C, -1, 0, 1, 2}, {CLS_C2C, -1, 0, 2, 2}, {CLS_C2C, -1, 0, 2, 2}};
+  const OpSpec b_ops[3] = {{CLS_C2R, 1, 0, 2, 1}, {CLS_C2C, 1, 0, 2, 2}, {CLS_C2C, 1, 0, 2, 2}};
+  const int f_order[3] = {0, 1, 2}, b_order[3] = {1, 2, 0};
+  Layout real, spec;
+  for (int a = 0; a < 3; a++) {
+    real.dims[a] = dims[a];
+    real.mo[a] = mo_real[a];
+    spec.dims[a] = a == 0 ? dims[0] / 2 + 1 : dims[a];
+    spec.mo[a] = mo_spec[a];
+  }
+  real.dt = 1;
+  spec.dt = 2;
+  if (!plan_chain(prec, f_ops, f_order, real, spec, fwd, batch_threads)) return false;
+  if (!plan_chain(prec, b_ops, b_order, spec, real, bwd, batch_threads)) return false;
+  if (fwd.passes.empty() || bwd.passes.empty()) {
+    set_error("Poisson plan has no pass");
+    return false;
+  }
+  PassDesc &pd = fwd.passes.back();
+  if (pd.lout.dt != 2) {
+    set_error("Poisson plan: last forward pass does not write the spectrum");
+    return false;
+  }
+  PassArgs &a = pd.args;
+  const int role_axis[3] = {pd.axis, pd.axis_e, pd.axis_f};
+  a.spec_op = 1;
+  for (int r = 0; r < 3; r++) {
+    a.kn[r] = dims[role_axis[r]];
+    a.khalf[r] = role_axis[r] == 0;
+  }
+  a.has_scale = 1;
+  a.scale = 1.0 / ((double)dims[0] * dims[1] * dims[2]);
   return true;
 }
 

@@ -33,7 +33,8 @@ struct Layout {
 
 struct PassDesc {
   PassArgs args;  // table pointers and in/out are filled by the executor
-  int axis;
+  int axis;              // logical axis d of the transform
+  int axis_e, axis_f;    // logical axes of the tile (e) and outer (f) roles
   Layout lin, lout;
   size_t in_bytes, out_bytes;
   size_t lds_bytes;
@@ -64,6 +65,14 @@ bool describe_pass(int prec, int axis, const OpSpec &op, const Layout &lin, cons
 // ops[a]: operation on logical axis a; order: axes in execution order.
 bool plan_chain(int prec, const OpSpec ops[3], const int order[3], const Layout &in, const Layout &out,
                 PassChain &chain, int batch_threads = 0);
+// The two chains of a periodic Poisson solve on a real grid of logical dims
+// (nx, ny, nz) in memory order mo_real: forward R2C x, C2C y, C2C z into the
+// spectrum (nx/2+1, ny, nz) in memory order mo_spec, whose last pass also
+// multiplies by -1/|k|^2 (0 at k = 0) and 1/(nx ny nz); backward C2C, C2C,
+// C2R x back to mo_real.  The domain is [0, 2 pi)^3, so wavenumbers are
+// integers.
+bool plan_poisson(int prec, const int dims[3], const int mo_real[3], const int mo_spec[3], PassChain &fwd,
+                  PassChain &bwd, int batch_threads = 0);
 // Re-tile a generic-path pass with B lines per tile and the given radix
 // sequence (the autotuner's knobs).  Returns false if the choice is invalid.
 bool retune_pass(int prec, PassDesc &ps, int B, const int *rad, int nrad);

@@ -317,6 +317,19 @@ P3E_HD P3E_INL void scatter(cx<T> *line, int n, cx<T> x, const PassArgs &p, cons
   }
 }
 
+// Poisson factor -1/|k|^2 (0 at k = 0) of output indices (kd, ke, kf).
+template <typename T> P3E_HD P3E_INL T poisson_factor(const PassArgs &p, int kd, int ke, int kf) {
+  const int idx[3] = {kd, ke, kf};
+  T s = (T)0;
+#pragma unroll
+  for (int r = 0; r < 3; r++) {
+    const int n = p.kn[r], i = idx[r];
+    const int k = (p.khalf[r] || 2 * i < n) ? i : i - n;
+    s += (T)k * (T)k;
+  }
+  return s > (T)0 ? (T)-1 / s : (T)0;
+}
+
 // One workgroup's work on one tile.  Phases are separated by bar(); the
 // kernel passes a workgroup barrier, the host test build runs each phase
 // with a single thread (tid 0, nth 1) and a no-op barrier.
@@ -411,6 +424,7 @@ P3E_HD void run_tile(const PassArgs &p, uint32_t tile, cx<T> *bufA, cx<T> *bufB,
         else kk = (T)(k - p.dg);
         y = mk(-kk * y.y, kk * y.x);
       }
+      if (p.spec_op) y = scl(y, poisson_factor<T>(p, k, e0 + j, (int)f));
       if (p.has_scale) y = scl(y, (T)p.scale);
       const long long off = base + (long long)k * p.os_d + (long long)j * p.os_e;
       if (p.out_cplx) {

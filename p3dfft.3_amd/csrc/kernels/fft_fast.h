@@ -298,6 +298,7 @@ P3E_HD void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, const cx<T
         else kk = (T)(k - p.dg);
         y = mk(-kk * y.y, kk * y.x);
       }
+      if (p.spec_op) y = scl(y, poisson_factor<T>(p, k, e0 + j, (int)f));
       if (p.has_scale) y = scl(y, (T)p.scale);
       out[base + (long long)k * p.os_d + (long long)j * p.os_e] = y;
     }

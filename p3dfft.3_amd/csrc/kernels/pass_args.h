@@ -74,6 +74,13 @@ struct PassArgs {
   int lp;            // LDS pitch of one line in the fast path (padded)
   const void *wN;    // R2C/C2R: M+1 entries exp(-2 pi i k / N)
   int nthreads;      // threads that work on a tile (a batch launch may have more)
+  // Spectral operator on the output (the Poisson service): 0 none; 1
+  // multiply by -1/|k|^2, 0 at k = 0.  The wavenumber of output index i along
+  // the pass's axis d, tile axis e and outer axis f (roles r = 0, 1, 2) is i
+  // if khalf[r] (the half spectrum of an R2C axis) or 2i < kn[r], else
+  // i - kn[r], with kn[r] the axis' logical real length.
+  int spec_op;
+  int kn[3], khalf[3];
 };
 
 // One entry of a batch work list: tile `tile` of the pass table[pass].

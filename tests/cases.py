@@ -47,3 +47,37 @@ def random_input(types, dims, mo, seed, hermitian_axis=None):
     if t0.dt1 == 1:
         return rng.uniform(-1.0, 1.0, shape)
     return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+def poisson_ref(dims, rhs, mo_real=(0, 1, 2), mo_spec=(1, 2, 0)):
+    """FP64 oracle Poisson solve on [0, 2 pi)^3: the oracle's R2C x / C2C y, z
+    transform, multiply by -1/|k|^2 (0 at k = 0) and 1/(nx ny nz), and the
+    oracle's inverse.  Returns (solution in mo_real storage, spectrum after
+    the multiply in mo_spec storage)."""
+    nx, ny, nz = dims
+    spec = orc.exec3d([R2C_D, CF_D, CF_D], list(dims), list(mo_real), list(mo_spec), np.asarray(rhs, np.float64))
+    lg = orc.from_storage(spec, list(mo_spec))
+    kx = np.arange(nx // 2 + 1, dtype=np.float64)
+    ky = np.fft.fftfreq(ny, 1.0 / ny)
+    kz = np.fft.fftfreq(nz, 1.0 / nz)
+    k2 = kx[:, None, None] ** 2 + ky[None, :, None] ** 2 + kz[None, None, :] ** 2
+    fac = np.zeros_like(k2)
+    fac[k2 > 0] = -1.0 / k2[k2 > 0]
+    spec2 = orc.to_storage(lg * fac / (nx * ny * nz), list(mo_spec))
+    sol = orc.exec3d([C2R_D, CB_D, CB_D], [nx // 2 + 1, ny, nz], list(mo_spec), list(mo_real), spec2)
+    return sol, spec2
+
+
+def gaussians_rhs(dims, seed, count=8):
+    """Sum of `count` random periodic Gaussians on [0, 2 pi)^3 minus its mean
+    (logical [x][y][z] array)."""
+    rng = np.random.default_rng(seed)
+    axes = [np.arange(n) * (2 * np.pi / n) for n in dims]
+    f = np.zeros(dims)
+    for _ in range(count):
+        c = rng.uniform(0, 2 * np.pi, 3)
+        w = rng.uniform(0.3, 0.8)
+        a = rng.uniform(-1, 1)
+        g = [np.exp(-(np.angle(np.exp(1j * (ax - ci))) ** 2) / (2 * w * w)) for ax, ci in zip(axes, c)]
+        f += a * g[0][:, None, None] * g[1][None, :, None] * g[2][None, None, :]
+    return f - f.mean()

@@ -94,6 +94,24 @@ template <typename T> static int run_pass(const PassDesc &pd, const void *in, vo
   return 0;
 }
 
+static void run_chain(int prec, const PassChain &ch, const void *in, void *out, int idir) {
+  std::vector<std::vector<char>> bufs(ch.passes.size());
+  const void *src = in;
+  for (size_t i = 0; i < ch.passes.size(); i++) {
+    const PassDesc &pd = ch.passes[i];
+    void *dst;
+    if (i + 1 == ch.passes.size()) dst = out;
+    else {
+      bufs[i].assign(pd.out_bytes, 0);
+      dst = bufs[i].data();
+    }
+    const int deriv = (idir >= 0 && idir == pd.axis && pd.deriv_capable) ? 1 : 0;
+    if (prec == 8) run_pass<double>(pd, src, dst, deriv);
+    else run_pass<float>(pd, src, dst, deriv);
+    src = dst;
+  }
+}
+
 extern "C" {
 
 // Plans and runs a transform on host arrays.  ops_*[a]: class, sign, r2r
@@ -125,21 +143,22 @@ int emu_exec(int prec, const int *cls, const int *sign, const int *kind, const i
     memcpy(out, in, out_bytes);
     return 0;
   }
-  std::vector<std::vector<char>> bufs(ch.passes.size());
-  const void *src = in;
-  for (size_t i = 0; i < ch.passes.size(); i++) {
-    const PassDesc &pd = ch.passes[i];
-    void *dst;
-    if (i + 1 == ch.passes.size()) dst = out;
-    else {
-      bufs[i].assign(pd.out_bytes, 0);
-      dst = bufs[i].data();
-    }
-    const int deriv = (idir >= 0 && idir == pd.axis && pd.deriv_capable) ? 1 : 0;
-    if (prec == 8) run_pass<double>(pd, src, dst, deriv);
-    else run_pass<float>(pd, src, dst, deriv);
-    src = dst;
+  run_chain(prec, ch, in, out, idir);
+  return 0;
+}
+
+// Periodic Poisson solve of a real grid through the product's Poisson plan
+// (forward chain with the fused -1/|k|^2 and 1/N multiply, backward chain).
+int emu_poisson(int prec, const int *dims, const int *mo_real, const int *mo_spec, const void *rhs, void *sol,
+                void *spec, char *desc, int desc_len, int batch_threads) {
+  PassChain f, b;
+  if (!plan_poisson(prec, dims, mo_real, mo_spec, f, b, batch_threads)) return -1;
+  if (desc && desc_len > 0) {
+    strncpy(desc, (f.desc + " || " + b.desc).c_str(), desc_len - 1);
+    desc[desc_len - 1] = 0;
   }
+  run_chain(prec, f, rhs, spec, -1);
+  run_chain(prec, b, spec, sol, -1);
   return 0;
 }
 

@@ -98,3 +98,22 @@ def tables_ready_counts(types, in_dims, mo1, out_dims, mo2, prec):
     if n < 0:
         raise RuntimeError(lib().emu_error().decode())
     return n, b.value, a.value
+
+
+def poisson(dims, rhs, prec, mo_real=(0, 1, 2), mo_spec=(1, 2, 0), batch_threads=0):
+    """Poisson solve through the product's Poisson plan on the CPU: returns
+    (solution in mo_real storage, spectrum after the -1/|k|^2 multiply in
+    mo_spec storage, plan description)."""
+    rdt = np.float64 if prec == 8 else np.float32
+    cdt = np.complex128 if prec == 8 else np.complex64
+    rhs = np.ascontiguousarray(rhs, dtype=rdt)
+    sol = np.zeros(orc.storage_shape(dims, mo_real), dtype=rdt)
+    sdims = [dims[0] // 2 + 1, dims[1], dims[2]]
+    spec = np.zeros(orc.storage_shape(sdims, mo_spec), dtype=cdt)
+    desc = ctypes.create_string_buffer(1024)
+    rc = lib().emu_poisson(prec, I3(*dims), I3(*mo_real), I3(*mo_spec), rhs.ctypes.data_as(ctypes.c_void_p),
+                           sol.ctypes.data_as(ctypes.c_void_p), spec.ctypes.data_as(ctypes.c_void_p), desc, 1024,
+                           int(batch_threads))
+    if rc != 0:
+        raise RuntimeError(lib().emu_error().decode())
+    return sol, spec, desc.value.decode()
