@@ -67,6 +67,39 @@ int p3dfft_hip_batch_set_graph(int bplan, int enable);
 int p3dfft_hip_batch_describe(int bplan, char *buf, int len);
 int p3dfft_hip_batch_destroy(int bplan);
 
+/* Periodic Poisson solves (domain [0, 2 pi)^3, grid nx x ny x nz of reals
+ * in memory order (0,1,2), x stride-1): u = inverse FFT of -f_k / |k|^2, with
+ * the k = 0 mode set to 0.  prec: 4 (FP32) or 8 (FP64).
+ *
+ * The service runs its own thread and stream.  Requests are admitted in
+ * arrival order when a slot (max_inflight) and their working memory (from a
+ * device pool of pool_bytes allocated at creation) are free; every admitted
+ * request advances one transform pass per step, and one step is one launch
+ * over the current pass of all requests in flight, so requests that arrive
+ * while others run join the next step.  rhs and sol are device arrays that
+ * the caller keeps alive until the request is polled as done; rhs must be
+ * complete when submitted.  A request's result does not depend on the other
+ * requests in flight.  Times are seconds since the service was created. */
+int p3dfft_hip_poisson_server_create(int prec, long long pool_bytes, int max_inflight);
+/* Plan a grid size ahead of its first request (plans and tables stay
+ * resident per size). */
+int p3dfft_hip_poisson_server_prepare(int server, const int dims[3]);
+/* Returns the request id, or -1 (e.g. a grid whose working memory exceeds
+ * the pool). */
+long long p3dfft_hip_poisson_submit(int server, const int dims[3], const void *rhs, void *sol);
+/* Writes up to cap completed requests (id, submit/admit/done times, ok flag)
+ * and returns their number; waits up to wait_ms for one if none is ready. */
+int p3dfft_hip_poisson_poll(int server, long long *ids, double *t_submit, double *t_admit, double *t_done,
+                            int *ok, int cap, int wait_ms);
+/* Steps launched and tiles processed so far; returns 1 if the service has
+ * failed, 0 if not, -1 for a bad handle. */
+int p3dfft_hip_poisson_server_stats(int server, long long *steps, long long *tiles);
+/* Waits for every submitted request, then frees the service. */
+int p3dfft_hip_poisson_server_destroy(int server);
+/* One solve on the regular per-transform path (same plan math, one request
+ * at a time); plans cached per size.  sync != 0 waits for the result. */
+int p3dfft_hip_poisson_solve(int prec, const int dims[3], const void *rhs, void *sol, void *stream, int sync);
+
 #ifdef __cplusplus
 }
 #endif

@@ -43,7 +43,7 @@ template <typename T> static void *upload_t(const std::vector<cld> &v, std::vect
   return d;
 }
 
-static bool upload_pass_tables(int prec, PassDesc &pd, std::vector<void *> &owned) {
+bool upload_pass_tables(int prec, PassDesc &pd, std::vector<void *> &owned) {
   PassArgs &a = pd.args;
   const std::vector<cld> *src[6] = {&pd.tw, &pd.chirp, &pd.chirp_hat, &pd.pre, &pd.post, &pd.wN};
   const void **dst[6] = {&a.tw, &a.chirp, &a.chirp_hat, &a.pre, &a.post, &a.wN};
@@ -60,7 +60,7 @@ static bool upload_pass_tables(int prec, PassDesc &pd, std::vector<void *> &owne
   return true;
 }
 
-static unsigned int persistent_grid(int prec, size_t lds, int nwork) {
+unsigned int persistent_grid(int prec, size_t lds, int nwork) {
   static int cus = 0;
   if (cus == 0) {
     int dev = 0;

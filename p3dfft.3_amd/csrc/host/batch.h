@@ -44,6 +44,13 @@ BatchPlan *batch_create(int prec, int nitems, const OpSpec ops[3], const int ord
 int batch_exec(BatchPlan *b, void *const *in, void *const *out, hipStream_t st, bool sync);
 void batch_destroy(BatchPlan *b);
 
+// Device copies of a pass's constant tables, owned by `owned`; sets the
+// table addresses in pd.args.
+bool upload_pass_tables(int prec, PassDesc &pd, std::vector<void *> &owned);
+// Workgroups of a persistent step-kernel launch over nwork tiles with the
+// given dynamic LDS: every CU filled to its occupancy, at most nwork.
+unsigned int persistent_grid(int prec, size_t lds, int nwork);
+
 // Autotuner cache file (P3DFFT_HIP_TUNE_CACHE, default
 // $HOME/.cache/p3dfft3_hip_tune.txt); returns the number of tuned keys.
 int tune_cache_size();

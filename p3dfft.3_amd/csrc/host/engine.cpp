@@ -93,6 +93,10 @@ static void free_tables(Pass &p) {
 Plan *plan_create(int prec, const OpSpec ops[3], const int order[3], const Layout &in, const Layout &out) {
   PassChain chain;
   if (!plan_chain(prec, ops, order, in, out, chain)) return nullptr;
+  return plan_from_chain(prec, chain, in, out);
+}
+
+Plan *plan_from_chain(int prec, PassChain &chain, const Layout &in, const Layout &out) {
   Plan *p = new Plan();
   p->prec = prec;
   p->lin = in;

@@ -42,6 +42,9 @@ int plan_profile_read(Plan *p, double *avg_ms, int cap, double *total_ms);
 // ops[a]: operation on logical axis a; order: axes in execution order (R2C
 // first, C2R last).  Returns nullptr on error (message in last_error()).
 Plan *plan_create(int prec, const OpSpec ops[3], const int order[3], const Layout &in, const Layout &out);
+// A plan over an already planned chain from layout `in` to `out` (uploads
+// its tables).
+Plan *plan_from_chain(int prec, PassChain &chain, const Layout &in, const Layout &out);
 // idir: logical axis of the spectral derivative, or -1.  in/out may be host
 // or device pointers; host arrays are staged and imply a synchronous call.
 int plan_exec(Plan *p, const void *in, void *out, int idir, bool OW, hipStream_t st, bool sync);

@@ -11,8 +11,9 @@
 namespace p3e {
 
 // ------------------------------------------------------------------ errors
-static char g_err[1024];
-static int g_err_code = 0;
+// per thread: the Poisson service runs its own thread
+static thread_local char g_err[1024];
+static thread_local int g_err_code = 0;
 
 void set_error(const char *fmt, ...) {
   va_list ap;

@@ -262,3 +262,70 @@ def batch_destroy(h):
     L = lib()
     L.p3dfft_hip_batch_destroy.argtypes = [ctypes.c_int]
     return L.p3dfft_hip_batch_destroy(h)
+
+
+# ------------------------------------------------------------ Poisson service
+
+def poisson_server_create(prec=4, pool_bytes=8 << 30, max_inflight=64):
+    L = lib()
+    L.p3dfft_hip_poisson_server_create.argtypes = [ctypes.c_int, ctypes.c_longlong, ctypes.c_int]
+    h = L.p3dfft_hip_poisson_server_create(int(prec), int(pool_bytes), int(max_inflight))
+    if h < 0:
+        raise RuntimeError(last_error())
+    return h
+
+
+def poisson_server_prepare(h, dims):
+    L = lib()
+    L.p3dfft_hip_poisson_server_prepare.argtypes = [ctypes.c_int, I3]
+    if L.p3dfft_hip_poisson_server_prepare(h, I3(*dims)) != 0:
+        raise RuntimeError(last_error())
+
+
+def poisson_submit(h, dims, rhs_ptr, sol_ptr):
+    """Submit one solve; rhs/sol are device addresses of nx*ny*nz reals in
+    memory order (0,1,2).  Returns the request id."""
+    L = lib()
+    L.p3dfft_hip_poisson_submit.argtypes = [ctypes.c_int, I3, VP, VP]
+    L.p3dfft_hip_poisson_submit.restype = ctypes.c_longlong
+    rid = L.p3dfft_hip_poisson_submit(h, I3(*dims), rhs_ptr, sol_ptr)
+    if rid < 0:
+        raise RuntimeError(last_error())
+    return rid
+
+
+def poisson_poll(h, cap=1024, wait_ms=0):
+    """[(id, t_submit, t_admit, t_done, ok)] of requests completed since the
+    last poll (seconds since the service started)."""
+    L = lib()
+    D = ctypes.c_double * cap
+    ids, ts, ta, td, ok = (ctypes.c_longlong * cap)(), D(), D(), D(), (ctypes.c_int * cap)()
+    L.p3dfft_hip_poisson_poll.argtypes = [ctypes.c_int, ctypes.c_longlong * cap, D, D, D, ctypes.c_int * cap,
+                                          ctypes.c_int, ctypes.c_int]
+    n = L.p3dfft_hip_poisson_poll(h, ids, ts, ta, td, ok, cap, int(wait_ms))
+    if n < 0:
+        raise RuntimeError(last_error())
+    return [(ids[i], ts[i], ta[i], td[i], bool(ok[i])) for i in range(n)]
+
+
+def poisson_server_stats(h):
+    L = lib()
+    L.p3dfft_hip_poisson_server_stats.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_longlong),
+                                                  ctypes.POINTER(ctypes.c_longlong)]
+    a, b = ctypes.c_longlong(), ctypes.c_longlong()
+    rc = L.p3dfft_hip_poisson_server_stats(h, ctypes.byref(a), ctypes.byref(b))
+    return {"steps": a.value, "tiles": b.value, "failed": rc == 1}
+
+
+def poisson_server_destroy(h):
+    L = lib()
+    L.p3dfft_hip_poisson_server_destroy.argtypes = [ctypes.c_int]
+    return L.p3dfft_hip_poisson_server_destroy(h)
+
+
+def poisson_solve(dims, rhs_ptr, sol_ptr, prec=4, stream=None, sync=True):
+    """One solve on the per-request path."""
+    L = lib()
+    L.p3dfft_hip_poisson_solve.argtypes = [ctypes.c_int, I3, VP, VP, VP, ctypes.c_int]
+    if L.p3dfft_hip_poisson_solve(int(prec), I3(*dims), rhs_ptr, sol_ptr, stream, int(bool(sync))) != 0:
+        raise RuntimeError(last_error())
