@@ -56,6 +56,212 @@ def cpu_baseline(n_sample=256):
                       f"rate = B_alg({n_sample}^3, FP32) / t"}
 
 
+def batch_bench(P, torch, reps=10):
+    """256 independent C2C FP32 3D transforms, each axis drawn from
+    {32, 48, 64, 96, 120, 128} (seed 4), complex64 N(0,1): one autotuned,
+    graph-replayed batch vs the per-transform path (one transform3D plan per
+    item, executed one after another on the library stream)."""
+    import numpy as np
+    rng = np.random.default_rng(4)
+    dims = rng.choice([32, 48, 64, 96, 120, 128], size=(256, 3)).tolist()
+    cf = P.type_id("CFFT_FORWARD_S")
+    gen = torch.Generator(device="cuda").manual_seed(4)
+    ins = [torch.complex(torch.randn(d[::-1], generator=gen, device="cuda"),
+                         torch.randn(d[::-1], generator=gen, device="cuda")) for d in dims]
+    outs = [torch.empty_like(t) for t in ins]
+    pts = sum(int(np.prod(d)) for d in dims)
+    bytes_fwd = 6 * 8 * pts  # one read + one write of complex64 per 1D stage
+    t0 = time.perf_counter()
+    h = P.batch_plan(dims, dims, [cf] * 3, tune=True)
+    t_plan = time.perf_counter() - t0
+    P.batch_set_graph(h, True)
+    st = P.stream()
+    ip, op = [t.data_ptr() for t in ins], [t.data_ptr() for t in outs]
+    for _ in range(2):
+        P.batch_exec(h, ip, op, st, sync=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        P.batch_exec(h, ip, op, st, sync=False)
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t0) / reps
+    # accuracy on a sample, against torch's FP64 FFT (the tests compare every item with the oracle)
+    errs = []
+    for i in range(0, 256, 37):
+        ref = torch.fft.fftn(ins[i].to(torch.complex128))
+        errs.append((torch.linalg.vector_norm(outs[i].to(torch.complex128) - ref) /
+                     torch.linalg.vector_norm(ref)).item())
+    # per-transform path
+    pg = P.init_proc_grid((1, 1, 1))
+    t3 = P.init_3Dtype((cf, cf, cf))
+    plans = {}
+    per = []
+    for d in dims:
+        k = tuple(d)
+        if k not in plans:
+            g = P.init_data_grid(d, -1, pg, (0, 1, 2), (0, 1, 2))
+            plans[k] = P.plan_3Dtrans(g, g, t3)
+        per.append(plans[k])
+    for _ in range(2):
+        for pl, a, b in zip(per, ip, op):
+            P.exec_async(pl, a, b, -1, 0, st, prec=4)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for pl, a, b in zip(per, ip, op):
+            P.exec_async(pl, a, b, -1, 0, st, prec=4)
+    torch.cuda.synchronize()
+    t_single = (time.perf_counter() - t0) / reps
+    P.batch_destroy(h)
+    log(f"batch: 256 items {t_batch * 1e3:.3f} ms ({bytes_fwd / t_batch / 1e9:.0f} GB/s), per-transform path "
+        f"{t_single * 1e3:.3f} ms, gain {t_single / t_batch:.2f}x, plan+tune {t_plan:.1f} s, max err {max(errs):.2e}")
+    return {"workload": "256 C2C FP32 3D transforms, axes from {32,48,64,96,120,128} (seed 4), one batch "
+                        "(autotuned, hipGraph replay) vs one transform3D exec per item",
+            "ms_per_batch": round(t_batch * 1e3, 4), "items_per_s": round(256 / t_batch, 1),
+            "GBps_alg": round(bytes_fwd / t_batch / 1e9, 1), "ms_per_transform_path": round(t_single * 1e3, 4),
+            "gain_vs_per_transform": round(t_single / t_batch, 3), "plan_and_tune_s": round(t_plan, 2),
+            "max_rel_l2_vs_fp64_sample": max(errs)}
+
+
+def poisson_shapes(count, seed):
+    """Grid edges log-uniform in [64, 384], rounded to multiples of 8, aspect at most 2:1."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    while len(out) < count:
+        d = [int(min(384, max(64, round(float(np.exp(rng.uniform(np.log(64), np.log(384)))) / 8) * 8)))
+             for _ in range(3)]
+        if max(d) <= 2 * min(d):
+            out.append(d)
+    return out
+
+
+def poisson_rhs(torch, d, seed):
+    """8 random periodic Gaussians minus their mean on [0, 2 pi)^3, storage (0,1,2) ([z][y][x])."""
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    ax = [torch.arange(n, device="cuda", dtype=torch.float64) * (2 * torch.pi / n) for n in d]
+    f = torch.zeros(d[::-1], device="cuda", dtype=torch.float64)
+    for _ in range(8):
+        c = torch.rand(3, generator=gen, device="cuda", dtype=torch.float64) * 2 * torch.pi
+        w = 0.3 + 0.5 * torch.rand(1, generator=gen, device="cuda", dtype=torch.float64)
+        a = 2 * torch.rand(1, generator=gen, device="cuda", dtype=torch.float64) - 1
+        g = [torch.exp(-torch.angle(torch.exp(1j * (ax[i] - c[i]))) ** 2 / (2 * w * w)) for i in range(3)]
+        f += a * g[2][:, None, None] * g[1][None, :, None] * g[0][None, None, :]
+    return (f - f.mean()).float().contiguous()
+
+
+def poisson_ref64(torch, rhs, d):
+    nx, ny, nz = d
+    F = torch.fft.rfftn(rhs.double(), dim=(2, 1, 0))  # storage [z][y][x]: rfft over x (last)
+    kz = torch.fft.fftfreq(nz, 1.0 / nz, device="cuda", dtype=torch.float64)
+    ky = torch.fft.fftfreq(ny, 1.0 / ny, device="cuda", dtype=torch.float64)
+    kx = torch.arange(nx // 2 + 1, device="cuda", dtype=torch.float64)
+    k2 = kz[:, None, None] ** 2 + ky[None, :, None] ** 2 + kx[None, None, :] ** 2
+    fac = torch.where(k2 > 0, -1.0 / torch.where(k2 > 0, k2, torch.ones_like(k2)), torch.zeros_like(k2))
+    return torch.fft.irfftn(F * fac, s=(nz, ny, nx), dim=(0, 1, 2))
+
+
+def serving_bench(P, torch, rates=(50, 200, 800), seconds=(4.0, 2.0, 1.0), nshapes=32, closed=256):
+    """Periodic Poisson solves (FP32) through the Poisson service: open-loop
+    Poisson arrivals at each rate (latency p50/p99, solves/s), and the
+    closed-loop capacity against the per-request path."""
+    import numpy as np
+    shapes = poisson_shapes(nshapes, 5)
+    rhs = [poisson_rhs(torch, d, 100 + i) for i, d in enumerate(shapes)]
+    h = P.poisson_server_create(4, 48 << 30, 64)
+    for d in shapes:
+        P.poisson_server_prepare(h, d)
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(6)
+    alg = [2 * (4 * int(np.prod(d)) + 5 * 8 * (d[0] // 2 + 1) * d[1] * d[2]) for d in shapes]
+
+    def drain(out, keep):
+        for rid, ts, ta, td, ok in P.poisson_poll(h, 4096, 0):
+            if not ok:
+                raise RuntimeError("a Poisson request failed")
+            out.append((rid, ts, td))
+            keep.pop(rid, None)
+
+    result = {"workload": "periodic Poisson solves, FP32, grid edges log-uniform in [64,384] (multiples of 8, "
+                          "aspect <= 2:1), 32 shapes (seed 5), rhs 8 Gaussians minus mean",
+              "rates": []}
+    # warm-up: every shape once
+    keep = {}
+    done = []
+    for i, d in enumerate(shapes):
+        o = torch.empty_like(rhs[i])
+        keep[P.poisson_submit(h, d, rhs[i].data_ptr(), o.data_ptr())] = o
+    while keep:
+        drain(done, keep)
+        time.sleep(0.001)
+    for rate, dur in zip(rates, seconds):
+        n = max(1, int(rate * dur))
+        t_arr = np.cumsum(rng.exponential(1.0 / rate, n))
+        which = rng.integers(0, nshapes, n)
+        keep, done, sub = {}, [], {}
+        t0 = time.perf_counter()
+        i = 0
+        while i < n or keep:
+            now = time.perf_counter() - t0
+            while i < n and t_arr[i] <= now:
+                k = int(which[i])
+                o = torch.empty_like(rhs[k])
+                rid = P.poisson_submit(h, shapes[k], rhs[k].data_ptr(), o.data_ptr())
+                keep[rid] = o
+                sub[rid] = k
+                i += 1
+            drain(done, keep)
+            if i < n and not keep:
+                time.sleep(max(0.0, min(0.0005, t_arr[i] - (time.perf_counter() - t0))))
+        lat = np.array([td - ts for _, ts, td in done]) * 1e3
+        span = max(td for _, _, td in done) - min(ts for _, ts, _ in done)
+        result["rates"].append({"offered_req_per_s": rate, "requests": n, "solves_per_s": round(n / span, 1),
+                                "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                                "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                                "mean_ms": round(float(lat.mean()), 3)})
+        log(f"serving at {rate} req/s: {n} requests, p50 {np.percentile(lat, 50):.2f} ms, "
+            f"p99 {np.percentile(lat, 99):.2f} ms")
+    # closed loop: `closed` requests submitted at once, vs the per-request path
+    which = rng.integers(0, nshapes, closed)
+    outs = [torch.empty_like(rhs[int(k)]) for k in which]
+    torch.cuda.synchronize()
+    keep, done = {}, []
+    t0 = time.perf_counter()
+    for k, o in zip(which, outs):
+        keep[P.poisson_submit(h, shapes[int(k)], rhs[int(k)].data_ptr(), o.data_ptr())] = o
+    while keep:
+        drain(done, keep)
+        time.sleep(0.0002)
+    t_srv = time.perf_counter() - t0
+    # accuracy of the service's results on a sample vs torch FP64
+    errs = []
+    for j in range(0, closed, 32):
+        k = int(which[j])
+        ref = poisson_ref64(torch, rhs[k], shapes[k])
+        errs.append((torch.linalg.vector_norm(outs[j].double() - ref) / torch.linalg.vector_norm(ref)).item())
+    st = P.stream()
+    for k in range(nshapes):  # plans of the per-request path
+        tmp = torch.empty_like(rhs[k])
+        P.poisson_solve(shapes[k], rhs[k].data_ptr(), tmp.data_ptr(), 4, st, True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k, o in zip(which, outs):
+        P.poisson_solve(shapes[int(k)], rhs[int(k)].data_ptr(), o.data_ptr(), 4, st, False)
+    torch.cuda.synchronize()
+    t_per = time.perf_counter() - t0
+    steps = P.poisson_server_stats(h)
+    P.poisson_server_destroy(h)
+    balg = sum(alg[int(k)] for k in which)
+    result.update({"closed_loop_requests": closed, "service_solves_per_s": round(closed / t_srv, 1),
+                   "per_request_path_solves_per_s": round(closed / t_per, 1),
+                   "gain_vs_per_request": round(t_per / t_srv, 3),
+                   "service_GBps_alg": round(balg / t_srv / 1e9, 1),
+                   "max_rel_l2_vs_fp64_sample": max(errs), "service_steps": steps["steps"]})
+    log(f"serving closed loop: service {closed / t_srv:.0f} solves/s, per-request path {closed / t_per:.0f} "
+        f"solves/s, gain {t_per / t_srv:.2f}x, max err {max(errs):.2e}")
+    return result
+
+
 def load_traffic():
     p = os.path.join(ROOT, "profiles", "traffic_pass_kernel_512cubed_fp32.json")
     if not os.path.exists(p):
@@ -71,6 +277,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 256-item batch measurement")
+    ap.add_argument("--no-serving", action="store_true", help="skip the Poisson service measurement")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed steps, run a plain 256 MiB device copy three times (the "
                          "known-size kernel that scales HBM byte counters under rocprofv3 --pmc)")
@@ -153,6 +361,8 @@ def main():
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle, 256^3 sample) ...")
         cpu = cpu_baseline()
+    batch = None if args.no_batch else batch_bench(P, torch)
+    serving = None if args.no_serving else serving_bench(P, torch)
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -177,6 +387,8 @@ def main():
                      "traffic": traffic, "bytes_per_launch_alg": per_launch_bytes,
                      "avg_launch_ms": round(avg_launch_ms, 5)},
         "cpu_baseline": cpu,
+        "batch": batch,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 
