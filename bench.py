@@ -152,7 +152,7 @@ def poisson_rhs(torch, d, seed):
 
 def poisson_ref64(torch, rhs, d):
     nx, ny, nz = d
-    F = torch.fft.rfftn(rhs.double(), dim=(2, 1, 0))  # storage [z][y][x]: rfft over x (last)
+    F = torch.fft.rfftn(rhs.double(), dim=(0, 1, 2))  # storage [z][y][x]: the half axis is x (last)
     kz = torch.fft.fftfreq(nz, 1.0 / nz, device="cuda", dtype=torch.float64)
     ky = torch.fft.fftfreq(ny, 1.0 / ny, device="cuda", dtype=torch.float64)
     kx = torch.arange(nx // 2 + 1, device="cuda", dtype=torch.float64)

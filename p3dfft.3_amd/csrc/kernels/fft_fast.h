@@ -105,96 +105,120 @@ P3E_HD P3E_INL void fast_tile_coords(const PassArgs &p, uint32_t tile, uint32_t 
 
 // Load phase: tile -> LDS (natural order; real lines packed two reals per
 // complex slot; for C2R the last input element X_M goes to a side slot).
-template <typename T, int M>
-P3E_HD void fast_load(const PassArgs &p, uint32_t f, int e0, int nb, cx<T> *buf, int tid, int nth) {
+// Split in two halves so that a kernel can issue the global loads of its
+// next tile (fast_fetch, into registers) before transforming the current
+// one, and write them to LDS (fast_put) afterwards.
+//
+// Every thread owns up to 2E reals or E complex of the tile (tot <= that *
+// nth by construction): entry q of thread tid is tile element
+// i = tid + q * nth.  Registers hold scalars (R2C: v[q] = real q; complex:
+// v[2q], v[2q+1] = re, im of q): arrays of cx<double> are not split into
+// registers by the compiler and would live in scratch.
+template <typename T> struct TileRegs {
+  T v[2 * P3E_FAST_E];
+};
+
+// (line j, element n) of tile entry i in load order.
+template <int M>
+P3E_HD P3E_INL void fast_load_index(const PassArgs &p, int i, int *j, int *n) {
   constexpr int logM = Log2<M>::v;
+  if (p.load_order == ORD_LINE) {
+    if (p.cls == CLS_R2C) {
+      *j = i >> (logM + 1);
+      *n = i & (2 * M - 1);
+    } else if (p.cls == CLS_C2R) {
+      *j = (int)fdiv((uint32_t)i, p.div_nin);
+      *n = i - *j * p.nin;
+    } else {
+      *j = i >> logM;
+      *n = i & (M - 1);
+    }
+  } else {
+    *j = i & (p.B - 1);
+    *n = i >> p.logB;
+  }
+}
+
+template <typename T, int M>
+P3E_HD P3E_INL void fast_fetch(const PassArgs &p, uint32_t f, int e0, int nb, TileRegs<T> &R, int tid, int nth) {
   constexpr int E = P3E_FAST_E;
-  cx<T> *side = buf + (size_t)p.B * p.lp;
   const long long base = (long long)e0 * p.is_e + (long long)f * p.is_f;
   const int tot = (p.load_order == ORD_LINE ? nb : p.B) * p.nin;
-  // Every thread owns up to 2E reals or E complex of the tile (tot <= that
-  // * nth by construction).  It issues CH complex (2CH real) global loads at
-  // a time before writing them to LDS, so that many loads are in flight.
-  // The staging arrays are arrays of scalars: arrays of cx<double> are not
-  // split into registers by the compiler and would live in scratch.
-  constexpr int CH = E;
+  // out-of-tile entries load the tile's first element (always valid) and
+  // are not written: unconditional loads keep the values in registers
   if (p.cls == CLS_R2C) {
     const T *in = reinterpret_cast<const T *>(p.in);
 #pragma unroll
-    for (int c = 0; c < 2 * E; c += 2 * CH) {
-    T r[2 * CH];
-    int slot[2 * CH];
-#pragma unroll
-    for (int q = 0; q < 2 * CH; q++) {
-      const int i = tid + (c + q) * nth;
-      int n, j;
-      if (p.load_order == ORD_LINE) {
-        j = i >> (logM + 1);
-        n = i & (2 * M - 1);
-      } else {
-        j = i & (p.B - 1);
-        n = i >> p.logB;
-      }
-      // out-of-tile entries load the tile's first element (always valid)
-      // and are not written: unconditional loads keep r[] in registers
+    for (int q = 0; q < 2 * E; q++) {
+      const int i = tid + q * nth;
+      int j, n;
+      fast_load_index<M>(p, i, &j, &n);
       const bool ok = i < tot && j < nb;
-      r[q] = in[ok ? base + (long long)n * p.is_d + (long long)j * p.is_e : base];
-      slot[q] = ok ? 2 * (j * p.lp + P3E_PAD(n >> 1)) + (n & 1) : -1;
-    }
-    T *bufr = reinterpret_cast<T *>(buf);
-#pragma unroll
-    for (int q = 0; q < 2 * CH; q++)
-      if (slot[q] >= 0) bufr[slot[q]] = r[q];
+      R.v[q] = in[ok ? base + (long long)n * p.is_d + (long long)j * p.is_e : base];
     }
   } else {
     const cx<T> *in = reinterpret_cast<const cx<T> *>(p.in);
 #pragma unroll
-    for (int c = 0; c < E; c += CH) {
-    T rx[CH], ry[CH];
-    int slot[CH];
-#pragma unroll
-    for (int q = 0; q < CH; q++) {
-      const int i = tid + (c + q) * nth;
-      int n, j;
-      if (p.load_order == ORD_LINE) {
-        if (p.cls == CLS_C2R) {
-          j = (int)fdiv((uint32_t)i, p.div_nin);
-          n = i - j * p.nin;
-        } else {
-          j = i >> logM;
-          n = i & (M - 1);
-        }
-      } else {
-        j = i & (p.B - 1);
-        n = i >> p.logB;
-      }
+    for (int q = 0; q < E; q++) {
+      const int i = tid + q * nth;
+      int j, n;
+      fast_load_index<M>(p, i, &j, &n);
       const bool ok = i < tot && j < nb;
-      const cx<T> v = in[ok ? base + (long long)n * p.is_d + (long long)j * p.is_e : base];
-      rx[q] = v.x;
-      ry[q] = v.y;
-      // C2R: X_M goes to the side slot
-      slot[q] = !ok ? -1 : (n < M) ? j * p.lp + P3E_PAD(n) : (int)(p.B * p.lp) + j;
-    }
-#pragma unroll
-    for (int q = 0; q < CH; q++)
-      if (slot[q] >= 0) buf[slot[q]] = mk(rx[q], ry[q]);
-    }
-    // C2R has B more elements (the X_M of each line) than E * nth
-    for (int i = tid + E * nth; i < tot; i += nth) {
-      int n, j;
-      if (p.load_order == ORD_LINE) {
-        j = (int)fdiv((uint32_t)i, p.div_nin);
-        n = i - j * p.nin;
-      } else {
-        j = i & (p.B - 1);
-        n = i >> p.logB;
-      }
-      if (j >= nb) continue;
-      const cx<T> x = in[base + (long long)n * p.is_d + (long long)j * p.is_e];
-      if (n < M) buf[j * p.lp + P3E_PAD(n)] = x;
-      else side[j] = x;
+      const cx<T> x = in[ok ? base + (long long)n * p.is_d + (long long)j * p.is_e : base];
+      R.v[2 * q] = x.x;
+      R.v[2 * q + 1] = x.y;
     }
   }
+}
+
+template <typename T, int M>
+P3E_HD P3E_INL void fast_put(const PassArgs &p, uint32_t f, int e0, int nb, const TileRegs<T> &R, cx<T> *buf,
+                             int tid, int nth) {
+  constexpr int E = P3E_FAST_E;
+  const int tot = (p.load_order == ORD_LINE ? nb : p.B) * p.nin;
+  if (p.cls == CLS_R2C) {
+    T *bufr = reinterpret_cast<T *>(buf);
+#pragma unroll
+    for (int q = 0; q < 2 * E; q++) {
+      const int i = tid + q * nth;
+      int j, n;
+      fast_load_index<M>(p, i, &j, &n);
+      if (i < tot && j < nb) bufr[2 * (j * p.lp + P3E_PAD(n >> 1)) + (n & 1)] = R.v[q];
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+      const int i = tid + q * nth;
+      int j, n;
+      fast_load_index<M>(p, i, &j, &n);
+      if (i < tot && j < nb) {
+        // C2R: X_M goes to the side slot after the B lines
+        const int slot = (n < M) ? j * p.lp + P3E_PAD(n) : (int)(p.B * p.lp) + j;
+        buf[slot] = mk(R.v[2 * q], R.v[2 * q + 1]);
+      }
+    }
+    // C2R has B more elements (the X_M of each line) than E * nth: loaded
+    // here directly
+    if (p.cls == CLS_C2R) {
+      const cx<T> *in = reinterpret_cast<const cx<T> *>(p.in);
+      const long long base = (long long)e0 * p.is_e + (long long)f * p.is_f;
+      for (int i = tid + E * nth; i < tot; i += nth) {
+        int j, n;
+        fast_load_index<M>(p, i, &j, &n);
+        if (j >= nb) continue;
+        const cx<T> x = in[base + (long long)n * p.is_d + (long long)j * p.is_e];
+        if (n < M) buf[j * p.lp + P3E_PAD(n)] = x;
+        else buf[(size_t)p.B * p.lp + j] = x;
+      }
+    }
+  }
+}
+
+template <typename T, int M>
+P3E_HD void fast_load(const PassArgs &p, uint32_t f, int e0, int nb, cx<T> *buf, int tid, int nth) {
+  TileRegs<T> R;
+  fast_fetch<T, M>(p, f, e0, nb, R, tid, nth);
+  fast_put<T, M>(p, f, e0, nb, R, buf, tid, nth);
 }
 
 // C2R: merge X_k and X_{M-k} into the packed spectrum, in place:
