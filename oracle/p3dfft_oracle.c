@@ -143,16 +143,27 @@ static void fft_rec(int n, const cx *in, long is, cx *out, const cx *w, long wst
   }
 }
 
+/* The roots of the last (n, sign) used by this thread: a 3D transform calls
+ * or_fft line after line with the same length, and the long-double roots
+ * dominate the cost otherwise.  Per thread, so concurrent callers are safe. */
+static _Thread_local cx *g_w = NULL;
+static _Thread_local int g_w_n = 0, g_w_sign = 0;
+
 void or_fft(int n, const double *in, double *out, int sign)
 {
   if (n <= 0) return;
-  cx *w = (cx *)malloc(sizeof(cx) * n);
+  if (g_w_n != n || g_w_sign != sign) {
+    free(g_w);
+    g_w = (cx *)malloc(sizeof(cx) * n);
+    for (int t = 0; t < n; t++) g_w[t] = root(t, n, sign);
+    g_w_n = n;
+    g_w_sign = sign;
+  }
   cx *tmp = (cx *)malloc(sizeof(cx) * n);
   cx *o = (cx *)malloc(sizeof(cx) * n);
-  for (int t = 0; t < n; t++) w[t] = root(t, n, sign);
-  fft_rec(n, (const cx *)in, 1, o, w, 1, tmp);
+  fft_rec(n, (const cx *)in, 1, o, g_w, 1, tmp);
   memcpy(out, o, sizeof(cx) * n);
-  free(w); free(tmp); free(o);
+  free(tmp); free(o);
 }
 
 /* FFTW r2c: first n/2+1 outputs of the forward DFT of real data. */
