@@ -104,7 +104,8 @@ P3E_HD P3E_INL void fast_tile_coords(const PassArgs &p, uint32_t tile, uint32_t 
 }
 
 // Load phase: tile -> LDS (natural order; real lines packed two reals per
-// complex slot; for C2R the last input element X_M goes to a side slot).
+// complex slot; C2R lines already merged into the packed half-length
+// spectrum: each thread loads the pairs (X_k, X_{M-k}) it merges).
 // Split in two halves so that a kernel can issue the global loads of its
 // next tile (fast_fetch, into registers) before transforming the current
 // one, and write them to LDS (fast_put) afterwards.
@@ -139,6 +140,33 @@ P3E_HD P3E_INL void fast_load_index(const PassArgs &p, int i, int *j, int *n) {
   }
 }
 
+// (line j, pair k) of C2R pair entry i: the pair is (X_k, X_{M-k}),
+// k = 0..M/2, along the line in load order.
+template <int M>
+P3E_HD P3E_INL void fast_pair_index(const PassArgs &p, int i, int *j, int *k) {
+  if (p.load_order == ORD_LINE) {
+    *j = (int)fdiv((uint32_t)i, p.div_pairs);
+    *k = i - *j * (M / 2 + 1);
+  } else {
+    *j = i & (p.B - 1);
+    *k = i >> p.logB;
+  }
+}
+
+// C2R: packed spectrum entries Z_k, Z_{M-k} of one line from X_k, X_{M-k}:
+//   Z_k = (X_k + conj X_{M-k}) + i e^{+2 pi i k/N} (X_k - conj X_{M-k});
+// the imaginary parts of X_0 and X_M do not contribute (FFTW c2r).
+template <typename T, int M>
+P3E_HD P3E_INL void c2r_pair_put(const PassArgs &p, cx<T> *line, int k, cx<T> xk, cx<T> xm) {
+  const cx<T> *wN = reinterpret_cast<const cx<T> *>(p.wN);
+  if (k == 0) {
+    line[P3E_PAD(0)] = mk(xk.x + xm.x, xk.x - xm.x);
+    return;
+  }
+  line[P3E_PAD(k)] = add(add(xk, conj(xm)), rot(mul(conj(wN[k]), sub(xk, conj(xm))), 1));
+  if (2 * k != M) line[P3E_PAD(M - k)] = add(add(xm, conj(xk)), rot(mul(conj(wN[M - k]), sub(xm, conj(xk))), 1));
+}
+
 template <typename T, int M>
 P3E_HD P3E_INL void fast_fetch(const PassArgs &p, uint32_t f, int e0, int nb, TileRegs<T> &R, int tid, int nth) {
   constexpr int E = P3E_FAST_E;
@@ -155,6 +183,24 @@ P3E_HD P3E_INL void fast_fetch(const PassArgs &p, uint32_t f, int e0, int nb, Ti
       fast_load_index<M>(p, i, &j, &n);
       const bool ok = i < tot && j < nb;
       R.v[q] = in[ok ? base + (long long)n * p.is_d + (long long)j * p.is_e : base];
+    }
+  } else if (p.cls == CLS_C2R) {
+    // pairs (X_k, X_{M-k}) of one line in one thread, merged in fast_put
+    const cx<T> *in = reinterpret_cast<const cx<T> *>(p.in);
+    const int ptot = (p.load_order == ORD_LINE ? nb : p.B) * (M / 2 + 1);
+#pragma unroll
+    for (int q = 0; q < E / 2; q++) {
+      const int i = tid + q * nth;
+      int j, k;
+      fast_pair_index<M>(p, i, &j, &k);
+      const bool ok = i < ptot && j < nb;
+      const long long o = base + (long long)j * p.is_e;
+      const cx<T> a = in[ok ? o + (long long)k * p.is_d : base];
+      const cx<T> b = in[ok ? o + (long long)(M - k) * p.is_d : base];
+      R.v[4 * q] = a.x;
+      R.v[4 * q + 1] = a.y;
+      R.v[4 * q + 2] = b.x;
+      R.v[4 * q + 3] = b.y;
     }
   } else {
     const cx<T> *in = reinterpret_cast<const cx<T> *>(p.in);
@@ -185,31 +231,33 @@ P3E_HD P3E_INL void fast_put(const PassArgs &p, uint32_t f, int e0, int nb, cons
       fast_load_index<M>(p, i, &j, &n);
       if (i < tot && j < nb) bufr[2 * (j * p.lp + P3E_PAD(n >> 1)) + (n & 1)] = R.v[q];
     }
+  } else if (p.cls == CLS_C2R) {
+    const int ptot = (p.load_order == ORD_LINE ? nb : p.B) * (M / 2 + 1);
+#pragma unroll
+    for (int q = 0; q < E / 2; q++) {
+      const int i = tid + q * nth;
+      int j, k;
+      fast_pair_index<M>(p, i, &j, &k);
+      if (i < ptot && j < nb)
+        c2r_pair_put<T, M>(p, buf + j * p.lp, k, mk(R.v[4 * q], R.v[4 * q + 1]), mk(R.v[4 * q + 2], R.v[4 * q + 3]));
+    }
+    // (M/2 + 1) B pairs are B more than (E/2) nth: loaded here directly
+    const cx<T> *in = reinterpret_cast<const cx<T> *>(p.in);
+    const long long base = (long long)e0 * p.is_e + (long long)f * p.is_f;
+    for (int i = tid + (E / 2) * nth; i < ptot; i += nth) {
+      int j, k;
+      fast_pair_index<M>(p, i, &j, &k);
+      if (j >= nb) continue;
+      const long long o = base + (long long)j * p.is_e;
+      c2r_pair_put<T, M>(p, buf + j * p.lp, k, in[o + (long long)k * p.is_d], in[o + (long long)(M - k) * p.is_d]);
+    }
   } else {
 #pragma unroll
     for (int q = 0; q < E; q++) {
       const int i = tid + q * nth;
       int j, n;
       fast_load_index<M>(p, i, &j, &n);
-      if (i < tot && j < nb) {
-        // C2R: X_M goes to the side slot after the B lines
-        const int slot = (n < M) ? j * p.lp + P3E_PAD(n) : (int)(p.B * p.lp) + j;
-        buf[slot] = mk(R.v[2 * q], R.v[2 * q + 1]);
-      }
-    }
-    // C2R has B more elements (the X_M of each line) than E * nth: loaded
-    // here directly
-    if (p.cls == CLS_C2R) {
-      const cx<T> *in = reinterpret_cast<const cx<T> *>(p.in);
-      const long long base = (long long)e0 * p.is_e + (long long)f * p.is_f;
-      for (int i = tid + E * nth; i < tot; i += nth) {
-        int j, n;
-        fast_load_index<M>(p, i, &j, &n);
-        if (j >= nb) continue;
-        const cx<T> x = in[base + (long long)n * p.is_d + (long long)j * p.is_e];
-        if (n < M) buf[j * p.lp + P3E_PAD(n)] = x;
-        else buf[(size_t)p.B * p.lp + j] = x;
-      }
+      if (i < tot && j < nb) buf[j * p.lp + P3E_PAD(n)] = mk(R.v[2 * q], R.v[2 * q + 1]);
     }
   }
 }
@@ -219,34 +267,6 @@ P3E_HD void fast_load(const PassArgs &p, uint32_t f, int e0, int nb, cx<T> *buf,
   TileRegs<T> R;
   fast_fetch<T, M>(p, f, e0, nb, R, tid, nth);
   fast_put<T, M>(p, f, e0, nb, R, buf, tid, nth);
-}
-
-// C2R: merge X_k and X_{M-k} into the packed spectrum, in place:
-//   Z_k = (X_k + conj X_{M-k}) + i e^{+2 pi i k/N} (X_k - conj X_{M-k}).
-template <typename T, int M>
-P3E_HD void fast_c2r_merge(const PassArgs &p, int nb, cx<T> *buf, int tid, int nth) {
-  const cx<T> *wN = reinterpret_cast<const cx<T> *>(p.wN);
-  const cx<T> *side = buf + (size_t)p.B * p.lp;
-  constexpr int half = M / 2;  // pairs (k, M-k) for k = 0..M/2
-  const int tot = nb * (half + 1);
-  for (int i = tid; i < tot; i += nth) {
-    const int j = i / (half + 1);
-    const int k = i - j * (half + 1);
-    cx<T> *l = buf + j * p.lp;
-    if (k == 0) {
-      // the imaginary parts of X_0 and X_M do not contribute (FFTW c2r)
-      const T a = l[P3E_PAD(0)].x, c = side[j].x;
-      l[P3E_PAD(0)] = mk(a + c, a - c);
-    } else {
-      const cx<T> xk = l[P3E_PAD(k)], xm = l[P3E_PAD(M - k)];
-      const cx<T> zk = add(add(xk, conj(xm)), rot(mul(conj(wN[k]), sub(xk, conj(xm))), 1));
-      if (k != M - k) {
-        const cx<T> zm = add(add(xm, conj(xk)), rot(mul(conj(wN[M - k]), sub(xm, conj(xk))), 1));
-        l[P3E_PAD(M - k)] = zm;
-      }
-      l[P3E_PAD(k)] = zk;
-    }
-  }
 }
 
 // Store phase: output rule and LDS -> tile.
@@ -342,10 +362,6 @@ P3E_HD void fast_tile(const PassArgs &p, uint32_t tile, cx<T> *buf, int nth, Run
   each([&](int tid) {
     if (tid < nth) fast_load<T, M>(p, f, e0, nb, buf, tid, nth);
   });
-  if (p.cls == CLS_C2R)
-    each([&](int tid) {
-      if (tid < nth) fast_c2r_merge<T, M>(p, nb, buf, tid, nth);
-    });
   fft_pow2<T, M>(run, buf, p.lp, nb, (p.cls == CLS_R2C) ? -1 : p.sign, reinterpret_cast<const cx<T> *>(p.tw));
   each([&](int tid) {
     if (tid < nth) fast_store<T, M>(p, f, e0, nb, buf, tid, nth);

@@ -74,6 +74,7 @@ struct PassArgs {
   int lp;            // LDS pitch of one line in the fast path (padded)
   const void *wN;    // R2C/C2R: M+1 entries exp(-2 pi i k / N)
   int nthreads;      // threads that work on a tile (a batch launch may have more)
+  FastDiv div_pairs; // C2R: division by M/2 + 1, the pairs (X_k, X_{M-k}) per line
   // Spectral operator on the output (the Poisson service): 0 none; 1
   // multiply by -1/|k|^2, 0 at k = 0.  The wavenumber of output index i along
   // the pass's axis d, tile axis e and outer axis f (roles r = 0, 1, 2) is i

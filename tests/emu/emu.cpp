@@ -49,7 +49,7 @@ struct HostEach {
 template <typename T, int M> static int run_pow2(const PassDesc &pd, const PassArgs &a) {
   HostPassRun<T> run{pd.threads, std::vector<cx<T>>((size_t)pd.threads * P3E_FAST_E)};
   HostEach each{pd.threads};
-  std::vector<cx<T>> buf((size_t)a.B * a.lp + a.B);
+  std::vector<cx<T>> buf((size_t)a.B * a.lp);
   for (unsigned int t = 0; t < pd.nblocks; t++) {
     for (auto &x : buf) x = cx<T>{(T)1e30, (T)-1e30};
     fast_tile<T, M>(a, t, buf.data(), pd.threads, run, each);
