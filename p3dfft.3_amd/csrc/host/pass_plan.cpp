@@ -167,15 +167,18 @@ static void setup_fast(int prec, const OpSpec &op, PassDesc &ps, int batch_threa
   // Tile width: when the tile axis e is strided on one side (tile-order load
   // or store), the contiguous run per row is B elements, so B is as large as
   // LDS and threads allow.  When both sides run along whole lines, B only
-  // sets the workgroup size; about 128 threads (several workgroups per CU,
+  // sets the workgroup size; about 256 threads (several workgroups per CU,
   // which overlap one another's load, transform and store phases) measured
-  // fastest for 512-point FP32 lines on MI355X (0.26 ms vs 0.31 ms at 512
+  // fastest for 512-point FP32 lines on MI355X (0.25 ms vs 0.26 ms at 512
   // threads for a 512^3 pass).
   const bool line_both = a.load_order == ORD_LINE && a.store_order == ORD_LINE;
   int Bcap = batch_threads > 0 ? 256 : 64;
-  if (batch_threads == 0 && line_both) Bcap = std::max(1, 128 * P3E_FAST_E / M);
+  if (batch_threads == 0 && line_both) Bcap = std::max(1, 256 * P3E_FAST_E / M);
   const int Bmax = std::min(Bcap, next_pow2(a.ne));
-  auto lds = [&](int b) { return (size_t)b * lp * esz; };
+  // the pass kernel keeps its twiddles (M) and split factors (M + 1) in LDS
+  // after the tile; the batch kernel reads them from global memory
+  const size_t tables = batch_threads > 0 ? 0 : (size_t)(M + (op.cls == CLS_C2C ? 0 : M + 1)) * esz;
+  auto lds = [&](int b) { return (size_t)b * lp * esz + tables; };
   while (B * 2 <= Bmax && lds(B * 2) <= LDS_BUDGET && (long)M * B * 2 / P3E_FAST_E <= maxth) B *= 2;
   if (lds(B) > LDS_MAX || (long)M * B / P3E_FAST_E > maxth) return;
   a.fast = 1;

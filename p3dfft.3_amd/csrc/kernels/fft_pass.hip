@@ -52,14 +52,30 @@ struct DevEach {
   }
 };
 
-// Power-of-two path: complex length M at compile time, one LDS buffer.
+// Power-of-two path: complex length M at compile time, one LDS buffer for
+// the tile, followed in LDS by the twiddle table (M entries) and, for
+// R2C/C2R, the split factors (M + 1 entries): the radix passes and the
+// split/merge read them from LDS instead of issuing global loads (which
+// would compete with the tile's stream for the memory pipeline).  The copy
+// is complete before the barrier that ends the load phase.
 template <typename T, int M>
-__global__ void __launch_bounds__(P3E_FAST_MAX_THREADS) pass_kernel_pow2(PassArgs p) {
+__global__ void __launch_bounds__(P3E_FAST_MAX_THREADS) pass_kernel_pow2(PassArgs p0) {
   extern __shared__ unsigned char smem_raw[];
   cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
-  DevPassRun<T> run{(int)threadIdx.x, (int)blockDim.x};
-  DevEach each{(int)threadIdx.x};
-  fast_tile<T, M>(p, blockIdx.x, buf, blockDim.x, run, each);
+  cx<T> *ltw = buf + (size_t)p0.B * p0.lp;
+  cx<T> *lwn = ltw + M;
+  const cx<T> *gtw = reinterpret_cast<const cx<T> *>(p0.tw);
+  const cx<T> *gwn = reinterpret_cast<const cx<T> *>(p0.wN);
+  const int tid = threadIdx.x, nth = blockDim.x;
+  for (int i = tid; i < M; i += nth) ltw[i] = gtw[i];
+  if (p0.cls != CLS_C2C)
+    for (int i = tid; i <= M; i += nth) lwn[i] = gwn[i];
+  PassArgs p = p0;
+  p.tw = ltw;
+  p.wN = lwn;
+  DevPassRun<T> run{tid, nth};
+  DevEach each{tid};
+  fast_tile<T, M>(p, blockIdx.x, buf, nth, run, each);
 }
 
 // Batch step: a grid-stride loop over a work list of (pass, tile) entries
