@@ -10,6 +10,8 @@
 
 extern "C" int p3e_launch_step(const void *table, const void *work, int nwork, int prec, size_t lds,
                                unsigned int grid, hipStream_t st);
+extern "C" int p3e_launch_pass(const p3e::PassArgs *a, int prec, size_t lds_bytes, int threads,
+                               unsigned int nblocks, hipStream_t st);
 
 namespace p3e {
 
@@ -25,10 +27,12 @@ static long long dims_key(const int d[3]) { return ((long long)d[0] << 40) | ((l
 
 static size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
-static PoissonPlan *make_plan(int prec, const int dims[3], int batch_threads) {
+static PoissonPlan *make_plan(int prec, const int dims[3]) {
+  const bool dedicated = (long long)dims[0] * dims[1] * dims[2] >= DEDICATED_POINTS;
   PassChain f, b;
-  if (!plan_poisson(prec, dims, MO_REAL, MO_SPEC, f, b, batch_threads)) return nullptr;
+  if (!plan_poisson(prec, dims, MO_REAL, MO_SPEC, f, b, dedicated ? 0 : P3E_STEP_THREADS)) return nullptr;
   PoissonPlan *p = new PoissonPlan();
+  p->dedicated = dedicated;
   for (int a = 0; a < 3; a++) p->dims[a] = dims[a];
   p->passes = f.passes;
   p->passes.insert(p->passes.end(), b.passes.begin(), b.passes.end());
@@ -147,34 +151,46 @@ static void fail_all(PoissonServer *s, const char *msg) {
 }
 
 // Builds and launches one step over the current pass of every active
-// request (s->mu not held; only the service thread touches `active`).
+// request (s->mu not held; only the service thread touches `active`): the
+// passes of large grids as their own launches, then one step-kernel launch
+// over the tiles of all the others.
 static int launch_step(PoissonServer *s, PoissonServer::Step &b) {
   const int prec = s->prec;
-  size_t ntab = s->active.size(), nwork = 0, lds = 0;
+  size_t ntab = 0, nwork = 0, lds = 0;
   for (PoissonRequest *r : s->active) {
+    if (r->plan->dedicated) continue;
     const PassDesc &pd = r->plan->passes[r->phase];
+    ntab++;
     nwork += pd.nblocks;
     lds = std::max(lds, pd.lds_bytes);
   }
-  if (!ensure_cap(b, ntab, nwork)) return -1;
-  size_t w = 0;
-  for (size_t i = 0; i < ntab; i++) {
-    PoissonRequest *r = s->active[i];
+  if (!ensure_cap(b, std::max<size_t>(ntab, 1), std::max<size_t>(nwork, 1))) return -1;
+  size_t w = 0, t_i = 0;
+  for (PoissonRequest *r : s->active) {
     const PoissonPlan *pl = r->plan;
     const int ph = r->phase, last = (int)pl->passes.size() - 1;
     char *w0 = s->pool + r->off, *w1 = w0 + pl->work_bytes;
-    PassArgs a = pl->passes[ph].args;
+    const PassDesc &pd = pl->passes[ph];
+    PassArgs a = pd.args;
     a.in = ph == 0 ? r->rhs : ((ph - 1) % 2 ? w1 : w0);
     a.out = ph == last ? r->sol : (ph % 2 ? w1 : w0);
     a.deriv = 0;
-    b.h_tab[i] = a;
-    for (unsigned t = 0; t < pl->passes[ph].nblocks; t++) b.h_work[w++] = WorkItem{(uint32_t)i, t};
+    if (pl->dedicated) {
+      if (p3e_launch_pass(&a, prec, pd.lds_bytes, pd.threads, pd.nblocks, s->st)) return -1;
+      s->ntiles += pd.nblocks;
+      continue;
+    }
+    b.h_tab[t_i] = a;
+    for (unsigned t = 0; t < pd.nblocks; t++) b.h_work[w++] = WorkItem{(uint32_t)t_i, t};
+    t_i++;
   }
-  if (hipMemcpyAsync(b.d_tab, b.h_tab, ntab * sizeof(PassArgs), hipMemcpyHostToDevice, s->st) != hipSuccess ||
-      hipMemcpyAsync(b.d_work, b.h_work, nwork * sizeof(WorkItem), hipMemcpyHostToDevice, s->st) != hipSuccess)
-    return -1;
-  const unsigned grid = persistent_grid(prec, lds, (int)nwork);
-  if (p3e_launch_step(b.d_tab, b.d_work, (int)nwork, prec, lds, grid, s->st)) return -1;
+  if (nwork) {
+    if (hipMemcpyAsync(b.d_tab, b.h_tab, ntab * sizeof(PassArgs), hipMemcpyHostToDevice, s->st) != hipSuccess ||
+        hipMemcpyAsync(b.d_work, b.h_work, nwork * sizeof(WorkItem), hipMemcpyHostToDevice, s->st) != hipSuccess)
+      return -1;
+    const unsigned grid = persistent_grid(prec, lds, (int)nwork);
+    if (p3e_launch_step(b.d_tab, b.d_work, (int)nwork, prec, lds, grid, s->st)) return -1;
+  }
   if (hipEventRecord(b.done, s->st) != hipSuccess) return -1;
   s->nsteps++;
   s->ntiles += (long long)nwork;
@@ -293,7 +309,7 @@ static PoissonPlan *plan_of(PoissonServer *s, const int dims[3]) {
   const long long k = dims_key(dims);
   auto it = s->plans.find(k);
   if (it != s->plans.end()) return it->second;
-  PoissonPlan *p = make_plan(s->prec, dims, P3E_STEP_THREADS);
+  PoissonPlan *p = make_plan(s->prec, dims);
   if (!p) return nullptr;
   if (2 * p->work_bytes > s->pool_bytes) {
     set_error("Poisson service: a %dx%dx%d solve needs %zu bytes of working memory, the pool has %zu", dims[0],

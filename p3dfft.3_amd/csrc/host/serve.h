@@ -28,11 +28,18 @@ namespace p3e {
 
 struct PoissonPlan {
   int dims[3];
+  // Grids of at least DEDICATED_POINTS points run each pass as its own
+  // launch of the per-pass kernels (one such pass fills the GPU); smaller
+  // grids share step-kernel launches.  The choice depends on the grid size
+  // only, so a request's arithmetic is the same alone and in any mix.
+  bool dedicated;
   std::vector<PassDesc> passes;  // forward then backward, tables on the device
   std::vector<void *> owned;
   size_t work_bytes;             // one of the two working buffers
   size_t real_bytes;
 };
+
+static const long long DEDICATED_POINTS = 8LL << 20;
 
 struct PoissonRequest {
   long long id;

@@ -156,3 +156,21 @@ def test_service_rejects_oversized_request():
             P.poisson_submit(h, [64, 64, 64], t.data_ptr(), o.data_ptr())
     finally:
         P.poisson_server_destroy(h)
+
+
+def test_service_large_grid_dedicated_launches():
+    """A grid of at least 8M points runs its passes as their own launches,
+    mixed with small requests in the same steps: parity with the oracle, and
+    the same bits alone and in the mix."""
+    big = [256, 256, 128]
+    log("case: service large grid", big)
+    alone, _, _ = run_server([(big, 30)], 4)
+    reqs = [(SIZES[0], 31), (big, 30), (SIZES[1], 32), (SIZES[2], 33)]
+    mix, _, st = run_server(reqs, 4)
+    log("  steps", st)
+    assert np.array_equal(alone[0], mix[1])
+    err = C.rel_l2(mix[1], oracle(big, 30))
+    log("  rel L2 vs oracle", err)
+    assert err < TOL[4]
+    for s, (d, seed) in zip(mix, reqs):
+        assert C.rel_l2(s, oracle(d, seed)) < TOL[4]
