@@ -132,6 +132,24 @@ static const char *cls_name(int c) {
   return "?";
 }
 
+// Divisions of the generic path's radix passes (see PassArgs::div_nbf).
+static void set_radix_divs(PassArgs &a) {
+  long ns = 1;
+  for (int i = 0; i < a.nrad; i++) {
+    a.div_nbf[i] = make_fastdiv((uint32_t)(a.L / a.radix[i]));
+    a.div_ns[i] = make_fastdiv((uint32_t)ns);
+    ns *= a.radix[i];
+  }
+}
+
+// LDS pitch of a generic-path line of L elements: room for the padded line,
+// rounded up to 1 mod 32 elements for long lines (consecutive lines then
+// start on different banks for the tile-order load and store).
+static int generic_pitch(int L) {
+  const int need = P3E_PAD(L - 1) + 1;
+  return L >= 32 ? ((need + 31) & ~31) + 1 : need;
+}
+
 static const size_t LDS_BUDGET = 80 * 1024;
 static const size_t LDS_MAX = 160 * 1024;
 
@@ -360,12 +378,18 @@ bool describe_pass(int prec, int axis, const OpSpec &op, const Layout &lin, cons
   a.load_order = (is[d] == 1 && nin > 1) ? ORD_LINE : ORD_TILE;
   a.store_order = (os[d] == 1 && nout > 1) ? ORD_LINE : ORD_TILE;
   const size_t esz = 2 * (size_t)prec;
-  a.lpad = L + ((L >= 16) ? 1 : 0);
+  a.lpad = generic_pitch(L);
+  set_radix_divs(a);
   const size_t per_line = 2 * (size_t)a.lpad * esz;
   int B = 1;
   const int Bmax = std::min(32, next_pow2(a.ne));
-  while (B * 2 <= Bmax && per_line * (B * 2) <= LDS_BUDGET) B *= 2;
-  if (per_line * B > LDS_MAX) {
+  // the standalone pass kernel keeps the L twiddles in LDS after the two
+  // line buffers; the batch kernel reads them from global memory
+  size_t twl = (batch_threads > 0 || op.cls == CLS_EMPTY) ? 0 : (size_t)L * esz;
+  if (per_line + twl > LDS_MAX) twl = 0;  // long lines: twiddles stay in global memory
+  a.tw_in_lds = twl > 0;
+  while (B * 2 <= Bmax && per_line * (B * 2) + twl <= LDS_BUDGET) B *= 2;
+  if (per_line * B + twl > LDS_MAX) {
     set_error("transform length %d (work length %d) exceeds the LDS capacity of one workgroup", N, L);
     return false;
   }
@@ -376,7 +400,7 @@ bool describe_pass(int prec, int axis, const OpSpec &op, const Layout &lin, cons
   a.div_nin = make_fastdiv((uint32_t)std::max(nin, 1));
   a.div_nout = make_fastdiv((uint32_t)std::max(nout, 1));
   a.div_ntile = make_fastdiv((uint32_t)a.ntile_e);
-  ps.lds_bytes = per_line * B;
+  ps.lds_bytes = per_line * B + twl;
   ps.threads = 256;
   a.nthreads = 256;
   const unsigned long long nbk = (unsigned long long)a.ntile_e * (unsigned long long)a.nf;
@@ -540,6 +564,7 @@ bool retune_pass(int prec, PassDesc &ps, int B, const int *rad, int nrad) {
   if (lds > LDS_MAX) return false;
   for (int i = 0; i < nrad; i++) a.radix[i] = rad[i];
   a.nrad = nrad;
+  set_radix_divs(a);
   a.B = B;
   a.logB = 0;
   while ((1 << a.logB) < B) a.logB++;

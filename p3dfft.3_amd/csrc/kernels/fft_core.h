@@ -169,22 +169,23 @@ template <typename T> P3E_HD P3E_INL cx<T> twid(const cx<T> *tw, int idx, int s)
   return s < 0 ? w : conj(w);
 }
 
-// One Stockham radix-R pass over nb lines: src -> dst.
+// One Stockham radix-R pass over nb lines: src -> dst (padded LDS lines).
+// dnbf, dns: division by L / R and by Ns.
 template <typename T, int R>
-P3E_HD P3E_INL void stockham_pass(const cx<T> *src, cx<T> *dst, int L, int lpad, int nb,
-                                              int Ns, int s, const cx<T> *tw, int tid, int nth) {
+P3E_HD P3E_INL void stockham_pass(const cx<T> *src, cx<T> *dst, int L, int lpad, int nb, int Ns, int s,
+                                  const cx<T> *tw, int tid, int nth, const FastDiv &dnbf, const FastDiv &dns) {
   const int nbf = L / R;
   const int tstride = L / (Ns * R);
   const int total = nb * nbf;
   for (int t = tid; t < total; t += nth) {
-    const int line = t / nbf;
+    const int line = (int)fdiv((uint32_t)t, dnbf);
     const int j = t - line * nbf;
     const cx<T> *sl = src + line * lpad;
     cx<T> *dl = dst + line * lpad;
-    const int k = j % Ns;
+    const int k = j - (int)fdiv((uint32_t)j, dns) * Ns;
     cx<T> v[R];
 #pragma unroll
-    for (int r = 0; r < R; r++) v[r] = sl[j + r * nbf];
+    for (int r = 0; r < R; r++) v[r] = sl[P3E_PAD(j + r * nbf)];
     if (Ns > 1) {
 #pragma unroll
       for (int r = 1; r < R; r++) v[r] = mul(v[r], twid(tw, k * r * tstride, s));
@@ -192,34 +193,34 @@ P3E_HD P3E_INL void stockham_pass(const cx<T> *src, cx<T> *dst, int L, int lpad,
     bfly<T, R>(v, s);
     const int base = (j - k) * R + k;
 #pragma unroll
-    for (int r = 0; r < R; r++) dl[base + r * Ns] = v[r];
+    for (int r = 0; r < R; r++) dl[P3E_PAD(base + r * Ns)] = v[r];
   }
 }
 
 // Generic odd radix (7..31): outputs computed one at a time from LDS.
 template <typename T>
-P3E_HD void stockham_pass_generic(const cx<T> *src, cx<T> *dst, int L, int lpad, int nb, int Ns,
-                                      int R, int s, const cx<T> *tw, int tid, int nth) {
+P3E_HD void stockham_pass_generic(const cx<T> *src, cx<T> *dst, int L, int lpad, int nb, int Ns, int R, int s,
+                                  const cx<T> *tw, int tid, int nth, const FastDiv &dnbf, const FastDiv &dns) {
   const int nbf = L / R;
   const int tstride = L / (Ns * R);
   const int wstride = L / R;
   const int total = nb * nbf;
   for (int t = tid; t < total; t += nth) {
-    const int line = t / nbf;
+    const int line = (int)fdiv((uint32_t)t, dnbf);
     const int j = t - line * nbf;
     const cx<T> *sl = src + line * lpad;
     cx<T> *dl = dst + line * lpad;
-    const int k = j % Ns;
+    const int k = j - (int)fdiv((uint32_t)j, dns) * Ns;
     const int base = (j - k) * R + k;
     for (int q = 0; q < R; q++) {
       cx<T> acc = mk((T)0, (T)0);
       for (int r = 0; r < R; r++) {
-        cx<T> x = sl[j + r * nbf];
+        cx<T> x = sl[P3E_PAD(j + r * nbf)];
         if (Ns > 1 && r > 0) x = mul(x, twid(tw, k * r * tstride, s));
         int e = (r * q) % R;
         acc = add(acc, mul(x, twid(tw, e * wstride, s)));
       }
-      dl[base + q * Ns] = acc;
+      dl[P3E_PAD(base + q * Ns)] = acc;
     }
   }
 }
@@ -231,14 +232,15 @@ P3E_HD cx<T> *lds_fft(cx<T> *a, cx<T> *b, const PassArgs &p, int nb, int s, cons
   int Ns = 1;
   for (int i = 0; i < p.nrad; i++) {
     const int R = p.radix[i];
+    const FastDiv &dn = p.div_nbf[i], &ds = p.div_ns[i];
     switch (R) {
-      case 2: stockham_pass<T, 2>(a, b, p.L, p.lpad, nb, Ns, s, tw, tid, nth); break;
-      case 3: stockham_pass<T, 3>(a, b, p.L, p.lpad, nb, Ns, s, tw, tid, nth); break;
-      case 4: stockham_pass<T, 4>(a, b, p.L, p.lpad, nb, Ns, s, tw, tid, nth); break;
-      case 5: stockham_pass<T, 5>(a, b, p.L, p.lpad, nb, Ns, s, tw, tid, nth); break;
-      case 8: stockham_pass<T, 8>(a, b, p.L, p.lpad, nb, Ns, s, tw, tid, nth); break;
-      case 16: stockham_pass<T, 16>(a, b, p.L, p.lpad, nb, Ns, s, tw, tid, nth); break;
-      default: stockham_pass_generic<T>(a, b, p.L, p.lpad, nb, Ns, R, s, tw, tid, nth); break;
+      case 2: stockham_pass<T, 2>(a, b, p.L, p.lpad, nb, Ns, s, tw, tid, nth, dn, ds); break;
+      case 3: stockham_pass<T, 3>(a, b, p.L, p.lpad, nb, Ns, s, tw, tid, nth, dn, ds); break;
+      case 4: stockham_pass<T, 4>(a, b, p.L, p.lpad, nb, Ns, s, tw, tid, nth, dn, ds); break;
+      case 5: stockham_pass<T, 5>(a, b, p.L, p.lpad, nb, Ns, s, tw, tid, nth, dn, ds); break;
+      case 8: stockham_pass<T, 8>(a, b, p.L, p.lpad, nb, Ns, s, tw, tid, nth, dn, ds); break;
+      case 16: stockham_pass<T, 16>(a, b, p.L, p.lpad, nb, Ns, s, tw, tid, nth, dn, ds); break;
+      default: stockham_pass_generic<T>(a, b, p.L, p.lpad, nb, Ns, R, s, tw, tid, nth, dn, ds); break;
     }
     bar();
     cx<T> *t = a;
@@ -254,7 +256,7 @@ P3E_HD cx<T> *lds_fft(cx<T> *a, cx<T> *b, const PassArgs &p, int nb, int s, cons
 template <typename T>
 P3E_HD P3E_INL void put(cx<T> *line, int m, cx<T> v, const PassArgs &p, const cx<T> *chirp) {
   if (p.bluestein) v = mul(v, chirp[m]);
-  line[m] = v;
+  line[P3E_PAD(m)] = v;
 }
 
 // Scatter input element n of a line into the work sequence (length W).
@@ -333,13 +335,15 @@ template <typename T> P3E_HD P3E_INL T poisson_factor(const PassArgs &p, int kd,
 // One workgroup's work on one tile.  Phases are separated by bar(); the
 // kernel passes a workgroup barrier, the host test build runs each phase
 // with a single thread (tid 0, nth 1) and a no-op barrier.
+// tw_table: the twiddle table to use (p.tw, or a copy in LDS).
 template <typename T, typename Bar>
-P3E_HD void run_tile(const PassArgs &p, uint32_t tile, cx<T> *bufA, cx<T> *bufB, int tid, int nth, Bar bar) {
+P3E_HD void run_tile(const PassArgs &p, uint32_t tile, cx<T> *bufA, cx<T> *bufB, int tid, int nth, Bar bar,
+                     const cx<T> *tw_table) {
   const uint32_t f = fdiv(tile, p.div_ntile);
   const int e0 = (int)(tile - f * p.ntile_e) * p.B;
   const int nb = (p.B < p.ne - e0) ? p.B : (p.ne - e0);
 
-  const cx<T> *tw = reinterpret_cast<const cx<T> *>(p.tw);
+  const cx<T> *tw = tw_table;
   const cx<T> *chirp = reinterpret_cast<const cx<T> *>(p.chirp);
   const cx<T> *chirp_hat = reinterpret_cast<const cx<T> *>(p.chirp_hat);
   const cx<T> *pre = reinterpret_cast<const cx<T> *>(p.pre);
@@ -350,7 +354,7 @@ P3E_HD void run_tile(const PassArgs &p, uint32_t tile, cx<T> *bufA, cx<T> *bufB,
     const int tot = nb * p.L;
     for (int i = tid; i < tot; i += nth) {
       const int line = i / p.L;
-      bufA[line * p.lpad + (i - line * p.L)] = mk((T)0, (T)0);
+      bufA[line * p.lpad + P3E_PAD(i - line * p.L)] = mk((T)0, (T)0);
     }
     bar();
   }
@@ -358,16 +362,17 @@ P3E_HD void run_tile(const PassArgs &p, uint32_t tile, cx<T> *bufA, cx<T> *bufB,
   // 2. load tile, scatter into the work sequences
   {
     const long long base = (long long)e0 * p.is_e + (long long)f * p.is_f;
-    const int tot = nb * p.nin;
+    const int tot = (p.load_order == ORD_LINE ? nb : p.B) * p.nin;
     for (int i = tid; i < tot; i += nth) {
       int n, j;
       if (p.load_order == ORD_LINE) {
         j = (int)fdiv((uint32_t)i, p.div_nin);
         n = i - j * p.nin;
       } else {
-        // nb may be < B on the last tile: index over nb, not B
-        j = i % nb;
-        n = i / nb;
+        // nb may be < B on the last tile: lines j >= nb are skipped
+        j = i & (p.B - 1);
+        n = i >> p.logB;
+        if (j >= nb) continue;
       }
       const long long off = base + (long long)n * p.is_d + (long long)j * p.is_e;
       cx<T> x;
@@ -392,7 +397,7 @@ P3E_HD void run_tile(const PassArgs &p, uint32_t tile, cx<T> *bufA, cx<T> *bufB,
       for (int i = tid; i < tot; i += nth) {
         const int line = i / p.L;
         const int m = i - line * p.L;
-        res[line * p.lpad + m] = mul(res[line * p.lpad + m], chirp_hat[m]);
+        res[line * p.lpad + P3E_PAD(m)] = mul(res[line * p.lpad + P3E_PAD(m)], chirp_hat[m]);
       }
       bar();
       cx<T> *other = (res == bufA) ? bufB : bufA;
@@ -403,18 +408,19 @@ P3E_HD void run_tile(const PassArgs &p, uint32_t tile, cx<T> *bufA, cx<T> *bufB,
   // 4. output rule + store
   {
     const long long base = (long long)e0 * p.os_e + (long long)f * p.os_f;
-    const int tot = nb * p.nout;
+    const int tot = (p.store_order == ORD_LINE ? nb : p.B) * p.nout;
     for (int i = tid; i < tot; i += nth) {
       int k, j;
       if (p.store_order == ORD_LINE) {
         j = (int)fdiv((uint32_t)i, p.div_nout);
         k = i - j * p.nout;
       } else {
-        j = i % nb;
-        k = i / nb;
+        j = i & (p.B - 1);
+        k = i >> p.logB;
+        if (j >= nb) continue;
       }
       const int m = k + p.post_off;
-      cx<T> y = res[j * p.lpad + m];
+      cx<T> y = res[j * p.lpad + P3E_PAD(m)];
       if (p.bluestein) y = mul(y, chirp[m]);
       if (p.post_mode) y = mul(y, post[k]);
       if (p.deriv) {

@@ -22,12 +22,20 @@ struct DevBarrier {
   __device__ void operator()() const { __syncthreads(); }
 };
 
+// Generic path: two LDS line buffers, then the L twiddles (copied from
+// global memory before the first barrier of the tile).
 template <typename T>
 __global__ void __launch_bounds__(256) pass_kernel(PassArgs p) {
   extern __shared__ unsigned char smem_raw[];
   cx<T> *bufA = reinterpret_cast<cx<T> *>(smem_raw);
   cx<T> *bufB = bufA + (size_t)p.B * p.lpad;
-  run_tile<T>(p, blockIdx.x, bufA, bufB, threadIdx.x, blockDim.x, DevBarrier());
+  const cx<T> *tw = reinterpret_cast<const cx<T> *>(p.tw);
+  if (p.tw_in_lds) {
+    cx<T> *ltw = bufB + (size_t)p.B * p.lpad;
+    for (int i = threadIdx.x; i < p.L; i += blockDim.x) ltw[i] = tw[i];
+    tw = ltw;
+  }
+  run_tile<T>(p, blockIdx.x, bufA, bufB, threadIdx.x, blockDim.x, DevBarrier(), tw);
 }
 
 // One radix pass of the power-of-two path for this thread: read half,
@@ -120,7 +128,8 @@ __global__ void __launch_bounds__(P3E_STEP_THREADS)
       }
     } else {
       cx<T> *bufB = bufA + (size_t)sp.B * sp.lpad;
-      run_tile<T>(sp, wi.tile, bufA, bufB, threadIdx.x, blockDim.x, DevBarrier());
+      run_tile<T>(sp, wi.tile, bufA, bufB, threadIdx.x, blockDim.x, DevBarrier(),
+                  reinterpret_cast<const cx<T> *>(sp.tw));
     }
   }
 }

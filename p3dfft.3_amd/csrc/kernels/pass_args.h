@@ -75,6 +75,10 @@ struct PassArgs {
   const void *wN;    // R2C/C2R: M+1 entries exp(-2 pi i k / N)
   int nthreads;      // threads that work on a tile (a batch launch may have more)
   FastDiv div_pairs; // C2R: division by M/2 + 1, the pairs (X_k, X_{M-k}) per line
+  int tw_in_lds;     // generic path: the standalone kernel copies tw to LDS
+  // generic path, radix pass i: division by L / radix[i] (butterflies per
+  // line) and by the pass's input stride Ns = radix[0] ... radix[i-1]
+  FastDiv div_nbf[P3E_MAX_RADIX_PASSES], div_ns[P3E_MAX_RADIX_PASSES];
   // Spectral operator on the output (the Poisson service): 0 none; 1
   // multiply by -1/|k|^2, 0 at k = 0.  The wavenumber of output index i along
   // the pass's axis d, tile axis e and outer axis f (roles r = 0, 1, 2) is i
@@ -114,7 +118,8 @@ static inline int fast_radices(int M, int *rad) {
   }
   return 0;
 }
-// padded LDS index of the fast path: one spare slot every 16 elements
+// padded LDS index within a line (both paths): one spare slot every 16
+// elements, so that strided accesses of a radix pass spread over the banks
 #define P3E_PAD(i) ((i) + ((i) >> 4))
 
 }  // namespace p3e

@@ -89,7 +89,7 @@ template <typename T> static int run_pass(const PassDesc &pd, const void *in, vo
     // poison LDS between tiles so stale data cannot hide a missing write
     for (auto &v : A) v = cx<T>{(T)1e30, (T)-1e30};
     for (auto &v : B) v = cx<T>{(T)-1e30, (T)1e30};
-    run_tile<T>(a, t, A.data(), B.data(), 0, 1, HostBarrier());
+    run_tile<T>(a, t, A.data(), B.data(), 0, 1, HostBarrier(), reinterpret_cast<const cx<T> *>(a.tw));
   }
   return 0;
 }
