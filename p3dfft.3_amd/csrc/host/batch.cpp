@@ -172,7 +172,7 @@ static float time_candidate(int prec, PassDesc &pd, void *din, void *dout, PassA
 
 static void tune_pass(int prec, PassDesc &pd, hipStream_t st) {
   PassArgs &a = pd.args;
-  if (a.fast || a.cls == CLS_EMPTY) return;
+  if (a.fast || a.mid || a.cls == CLS_EMPTY) return;
   if (!tables_ready(pd)) return;  // never launch a pass whose tables are not on the device
   load_cache();
   const std::string key = tune_key(prec, pd);

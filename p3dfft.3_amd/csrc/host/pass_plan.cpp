@@ -158,6 +158,44 @@ void set_fast_path(bool enable) { g_fast_enabled = enable; }
 
 static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
+// Switch an already described generic pass to the single-buffer mixed-radix
+// path (fft_mid.h) when every radix is 2, 3, 4, 5, 8 or 16 and no Bluestein
+// is needed.  Tile: B lines with B * L <= 16 threads' worth of elements, as
+// large as LDS and the workgroup size (512 threads, or the batch kernel's)
+// allow; the twiddles go to LDS after the lines when they fit.
+static void setup_mid(int prec, const OpSpec &op, PassDesc &ps, int batch_threads) {
+  PassArgs &a = ps.args;
+  if (a.bluestein || op.cls == CLS_EMPTY || a.nrad < 1) return;
+  for (int i = 0; i < a.nrad; i++) {
+    const int r = a.radix[i];
+    if (r != 2 && r != 3 && r != 4 && r != 5 && r != 8 && r != 16) return;
+  }
+  const int L = a.L;
+  const long maxth = batch_threads > 0 ? batch_threads : 512;
+  const size_t esz = 2 * (size_t)prec;
+  const size_t per_line = (size_t)a.lpad * esz;
+  size_t twl = batch_threads > 0 ? 0 : (size_t)L * esz;
+  if (per_line + twl > LDS_MAX) twl = 0;
+  int B = 1;
+  const int Bmax = std::min(32, next_pow2(a.ne));
+  while (B * 2 <= Bmax && per_line * (B * 2) + twl <= LDS_BUDGET && (long)B * 2 * L <= 16 * maxth) B *= 2;
+  if (per_line * B + twl > LDS_MAX || (long)B * L > 16 * maxth) return;
+  int th = (int)(((long)B * L + 15) / 16);
+  th = std::max(64, (th + 63) & ~63);
+  if (th > maxth) return;
+  a.mid = 1;
+  a.tw_in_lds = twl > 0;
+  a.B = B;
+  a.logB = 0;
+  while ((1 << a.logB) < B) a.logB++;
+  a.ntile_e = (a.ne + B - 1) / B;
+  a.div_ntile = make_fastdiv((uint32_t)a.ntile_e);
+  ps.lds_bytes = per_line * B + twl;
+  ps.threads = th;
+  a.nthreads = th;
+  ps.nblocks = (unsigned int)((unsigned long long)a.ntile_e * (unsigned long long)a.nf);
+}
+
 // Switch an already described pass to the power-of-two path when it is a
 // C2C, R2C or C2R of power-of-two complex length M in [16, 4096] (M = N/2 for
 // the real transforms).  Tile: B lines, B as large as the LDS budget and
@@ -200,6 +238,8 @@ static void setup_fast(int prec, const OpSpec &op, PassDesc &ps, int batch_threa
   while (B * 2 <= Bmax && lds(B * 2) <= LDS_BUDGET && (long)M * B * 2 / P3E_FAST_E <= maxth) B *= 2;
   if (lds(B) > LDS_MAX || (long)M * B / P3E_FAST_E > maxth) return;
   a.fast = 1;
+  a.mid = 0;
+  a.tw_in_lds = 0;
   a.M = M;
   a.logM = 0;
   while ((1 << a.logM) < M) a.logM++;
@@ -418,6 +458,7 @@ bool describe_pass(int prec, int axis, const OpSpec &op, const Layout &lin, cons
   a.dg = (op.cls == CLS_R2C) ? 2 * (nout - 1) : nout;
   a.dmid = a.dg / 2;
   a.scale = 1.0;
+  setup_mid(prec, op, ps, batch_threads);
   setup_fast(prec, op, ps, batch_threads);
   return true;
 }
@@ -532,7 +573,7 @@ bool plan_chain(int prec, const OpSpec ops[3], const int order[3], const Layout 
     chain.pass_of_axis[axes[i]] = (int)chain.passes.size();
     snprintf(buf, sizeof(buf), "%s%s[axis %d, N=%d%s, mo (%d%d%d)->(%d%d%d), B=%d]", i ? " | " : "",
              cls_name(ops[axes[i]].cls), axes[i], ps.args.N,
-             ps.args.bluestein ? ", bluestein" : (ps.args.fast ? ", pow2" : ""),
+             ps.args.bluestein ? ", bluestein" : (ps.args.fast ? ", pow2" : (ps.args.mid ? ", mid" : "")),
              lay[i].mo[0], lay[i].mo[1], lay[i].mo[2], lay[i + 1].mo[0], lay[i + 1].mo[1], lay[i + 1].mo[2],
              ps.args.B);
     chain.desc += buf;
@@ -552,7 +593,8 @@ static bool radix_supported(int r) {
 
 bool retune_pass(int prec, PassDesc &ps, int B, const int *rad, int nrad) {
   PassArgs &a = ps.args;
-  if (a.fast || a.cls == CLS_EMPTY || B < 1 || (B & (B - 1)) || nrad < 0 || nrad > P3E_MAX_RADIX_PASSES)
+  if (a.fast || a.mid || a.cls == CLS_EMPTY || B < 1 || (B & (B - 1)) || nrad < 0 ||
+      nrad > P3E_MAX_RADIX_PASSES)
     return false;
   long prod = 1;
   for (int i = 0; i < nrad; i++) {

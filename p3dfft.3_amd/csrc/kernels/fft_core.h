@@ -332,67 +332,118 @@ template <typename T> P3E_HD P3E_INL T poisson_factor(const PassArgs &p, int kd,
   return s > (T)0 ? (T)-1 / s : (T)0;
 }
 
-// One workgroup's work on one tile.  Phases are separated by bar(); the
-// kernel passes a workgroup barrier, the host test build runs each phase
-// with a single thread (tid 0, nth 1) and a no-op barrier.
-// tw_table: the twiddle table to use (p.tw, or a copy in LDS).
+// Phases of one tile of the generic path, shared with the single-buffer
+// mixed-radix path (fft_mid.h).
+
+P3E_HD P3E_INL void generic_tile_coords(const PassArgs &p, uint32_t tile, uint32_t *f, int *e0, int *nb) {
+  *f = fdiv(tile, p.div_ntile);
+  *e0 = (int)(tile - *f * p.ntile_e) * p.B;
+  *nb = (p.B < p.ne - *e0) ? p.B : (p.ne - *e0);
+}
+
+// Clear the lines when the scatter does not write every work element.
+template <typename T>
+P3E_HD P3E_INL void generic_zero(const PassArgs &p, int nb, cx<T> *bufA, int tid, int nth) {
+  const int tot = nb * p.L;
+  for (int i = tid; i < tot; i += nth) {
+    const int line = i / p.L;
+    bufA[line * p.lpad + P3E_PAD(i - line * p.L)] = mk((T)0, (T)0);
+  }
+}
+
+// Load the tile and scatter each element into its line's work sequence.
+template <typename T>
+P3E_HD P3E_INL void generic_load(const PassArgs &p, uint32_t f, int e0, int nb, cx<T> *bufA, int tid, int nth) {
+  const cx<T> *chirp = reinterpret_cast<const cx<T> *>(p.chirp);
+  const cx<T> *pre = reinterpret_cast<const cx<T> *>(p.pre);
+  const long long base = (long long)e0 * p.is_e + (long long)f * p.is_f;
+  const int tot = (p.load_order == ORD_LINE ? nb : p.B) * p.nin;
+  for (int i = tid; i < tot; i += nth) {
+    int n, j;
+    if (p.load_order == ORD_LINE) {
+      j = (int)fdiv((uint32_t)i, p.div_nin);
+      n = i - j * p.nin;
+    } else {
+      // nb may be < B on the last tile: lines j >= nb are skipped
+      j = i & (p.B - 1);
+      n = i >> p.logB;
+      if (j >= nb) continue;
+    }
+    const long long off = base + (long long)n * p.is_d + (long long)j * p.is_e;
+    cx<T> x;
+    if (p.in_cplx) {
+      x = reinterpret_cast<const cx<T> *>(p.in)[off];
+    } else {
+      x = mk(reinterpret_cast<const T *>(p.in)[off], (T)0);
+    }
+    scatter(bufA + j * p.lpad, n, x, p, chirp, pre);
+  }
+}
+
+// Output rule (chirp, post-twiddle, derivative, spectral operator, scale)
+// and store of the transformed lines in `res`.
+template <typename T>
+P3E_HD P3E_INL void generic_store(const PassArgs &p, uint32_t f, int e0, int nb, const cx<T> *res, int tid,
+                                  int nth) {
+  const cx<T> *chirp = reinterpret_cast<const cx<T> *>(p.chirp);
+  const cx<T> *post = reinterpret_cast<const cx<T> *>(p.post);
+  const long long base = (long long)e0 * p.os_e + (long long)f * p.os_f;
+  const int tot = (p.store_order == ORD_LINE ? nb : p.B) * p.nout;
+  for (int i = tid; i < tot; i += nth) {
+    int k, j;
+    if (p.store_order == ORD_LINE) {
+      j = (int)fdiv((uint32_t)i, p.div_nout);
+      k = i - j * p.nout;
+    } else {
+      j = i & (p.B - 1);
+      k = i >> p.logB;
+      if (j >= nb) continue;
+    }
+    const int m = k + p.post_off;
+    cx<T> y = res[j * p.lpad + P3E_PAD(m)];
+    if (p.bluestein) y = mul(y, chirp[m]);
+    if (p.post_mode) y = mul(y, post[k]);
+    if (p.deriv) {
+      T kk;
+      if (k < p.dmid) kk = (T)k;
+      else if (k == p.dmid) kk = (T)0;
+      else kk = (T)(k - p.dg);
+      y = mk(-kk * y.y, kk * y.x);
+    }
+    if (p.spec_op) y = scl(y, poisson_factor<T>(p, k, e0 + j, (int)f));
+    if (p.has_scale) y = scl(y, (T)p.scale);
+    const long long off = base + (long long)k * p.os_d + (long long)j * p.os_e;
+    if (p.out_cplx) {
+      reinterpret_cast<cx<T> *>(p.out)[off] = y;
+    } else {
+      reinterpret_cast<T *>(p.out)[off] = y.x;
+    }
+  }
+}
+
+// One workgroup's work on one tile of the generic (two-buffer) path.
+// Phases are separated by bar(); the host test build runs each phase with a
+// single thread (tid 0, nth 1) and a no-op barrier.  tw_table: the twiddle
+// table to use (p.tw, or a copy in LDS).
 template <typename T, typename Bar>
 P3E_HD void run_tile(const PassArgs &p, uint32_t tile, cx<T> *bufA, cx<T> *bufB, int tid, int nth, Bar bar,
                      const cx<T> *tw_table) {
-  const uint32_t f = fdiv(tile, p.div_ntile);
-  const int e0 = (int)(tile - f * p.ntile_e) * p.B;
-  const int nb = (p.B < p.ne - e0) ? p.B : (p.ne - e0);
-
-  const cx<T> *tw = tw_table;
-  const cx<T> *chirp = reinterpret_cast<const cx<T> *>(p.chirp);
+  uint32_t f;
+  int e0, nb;
+  generic_tile_coords(p, tile, &f, &e0, &nb);
   const cx<T> *chirp_hat = reinterpret_cast<const cx<T> *>(p.chirp_hat);
-  const cx<T> *pre = reinterpret_cast<const cx<T> *>(p.pre);
-  const cx<T> *post = reinterpret_cast<const cx<T> *>(p.post);
-
-  // 1. clear lines when the scatter does not write every work element
   if (p.zero_fill) {
-    const int tot = nb * p.L;
-    for (int i = tid; i < tot; i += nth) {
-      const int line = i / p.L;
-      bufA[line * p.lpad + P3E_PAD(i - line * p.L)] = mk((T)0, (T)0);
-    }
+    generic_zero<T>(p, nb, bufA, tid, nth);
     bar();
   }
-
-  // 2. load tile, scatter into the work sequences
-  {
-    const long long base = (long long)e0 * p.is_e + (long long)f * p.is_f;
-    const int tot = (p.load_order == ORD_LINE ? nb : p.B) * p.nin;
-    for (int i = tid; i < tot; i += nth) {
-      int n, j;
-      if (p.load_order == ORD_LINE) {
-        j = (int)fdiv((uint32_t)i, p.div_nin);
-        n = i - j * p.nin;
-      } else {
-        // nb may be < B on the last tile: lines j >= nb are skipped
-        j = i & (p.B - 1);
-        n = i >> p.logB;
-        if (j >= nb) continue;
-      }
-      const long long off = base + (long long)n * p.is_d + (long long)j * p.is_e;
-      cx<T> x;
-      if (p.in_cplx) {
-        x = reinterpret_cast<const cx<T> *>(p.in)[off];
-      } else {
-        x = mk(reinterpret_cast<const T *>(p.in)[off], (T)0);
-      }
-      scatter(bufA + j * p.lpad, n, x, p, chirp, pre);
-    }
-  }
+  generic_load<T>(p, f, e0, nb, bufA, tid, nth);
   bar();
-
-  // 3. FFT
   cx<T> *res = bufA;
   if (p.cls != CLS_EMPTY) {
     if (!p.bluestein) {
-      res = lds_fft<T>(bufA, bufB, p, nb, p.sign, tw, tid, nth, bar);
+      res = lds_fft<T>(bufA, bufB, p, nb, p.sign, tw_table, tid, nth, bar);
     } else {
-      res = lds_fft<T>(bufA, bufB, p, nb, -1, tw, tid, nth, bar);
+      res = lds_fft<T>(bufA, bufB, p, nb, -1, tw_table, tid, nth, bar);
       const int tot = nb * p.L;
       for (int i = tid; i < tot; i += nth) {
         const int line = i / p.L;
@@ -401,46 +452,10 @@ P3E_HD void run_tile(const PassArgs &p, uint32_t tile, cx<T> *bufA, cx<T> *bufB,
       }
       bar();
       cx<T> *other = (res == bufA) ? bufB : bufA;
-      res = lds_fft<T>(res, other, p, nb, +1, tw, tid, nth, bar);
+      res = lds_fft<T>(res, other, p, nb, +1, tw_table, tid, nth, bar);
     }
   }
-
-  // 4. output rule + store
-  {
-    const long long base = (long long)e0 * p.os_e + (long long)f * p.os_f;
-    const int tot = (p.store_order == ORD_LINE ? nb : p.B) * p.nout;
-    for (int i = tid; i < tot; i += nth) {
-      int k, j;
-      if (p.store_order == ORD_LINE) {
-        j = (int)fdiv((uint32_t)i, p.div_nout);
-        k = i - j * p.nout;
-      } else {
-        j = i & (p.B - 1);
-        k = i >> p.logB;
-        if (j >= nb) continue;
-      }
-      const int m = k + p.post_off;
-      cx<T> y = res[j * p.lpad + P3E_PAD(m)];
-      if (p.bluestein) y = mul(y, chirp[m]);
-      if (p.post_mode) y = mul(y, post[k]);
-      if (p.deriv) {
-        T kk;
-        if (k < p.dmid) kk = (T)k;
-        else if (k == p.dmid) kk = (T)0;
-        else kk = (T)(k - p.dg);
-        y = mk(-kk * y.y, kk * y.x);
-      }
-      if (p.spec_op) y = scl(y, poisson_factor<T>(p, k, e0 + j, (int)f));
-      if (p.has_scale) y = scl(y, (T)p.scale);
-      const long long off = base + (long long)k * p.os_d + (long long)j * p.os_e;
-      if (p.out_cplx) {
-        reinterpret_cast<cx<T> *>(p.out)[off] = y;
-      } else {
-        reinterpret_cast<T *>(p.out)[off] = y.x;
-      }
-    }
-  }
+  generic_store<T>(p, f, e0, nb, res, tid, nth);
 }
-
 
 }  // namespace p3e

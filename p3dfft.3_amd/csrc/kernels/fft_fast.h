@@ -27,6 +27,7 @@ template <> struct Log2<1> {
 // Radix-R Stockham pass at stride Ns of a length-M FFT, split in halves.
 template <typename T, int R, int M, int Ns> struct FPass {
   static constexpr int E = P3E_FAST_E;
+  static constexpr int SLOTS = E;  // registers per thread between the halves
   static constexpr int NB = E / R;  // butterflies per thread
   static constexpr int logR = Log2<R>::v;
   static constexpr int lognbf = Log2<M>::v - logR;

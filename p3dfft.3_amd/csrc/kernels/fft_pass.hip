@@ -15,12 +15,24 @@
 
 #include "fft_core.h"
 #include "fft_fast.h"
+#include "fft_mid.h"
 
 namespace p3e {
 
 struct DevBarrier {
   __device__ void operator()() const { __syncthreads(); }
 };
+
+// Workgroups are dispatched to the 8 XCDs of an MI355X round-robin (block b
+// to XCD b % 8), and each XCD has its own L2.  Tiles that are adjacent along
+// the tile axis share cache lines when their rows are shorter than a line,
+// so consecutive tiles are given to workgroups of the same XCD: block b
+// takes tile (b % 8) * (G / 8) + b / 8 (the last G % 8 blocks keep their
+// index).  A bijection on [0, G).
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t G) {
+  const uint32_t per = G >> 3;
+  return b < (per << 3) ? (b & 7u) * per + (b >> 3) : b;
+}
 
 // Generic path: two LDS line buffers, then the L twiddles (copied from
 // global memory before the first barrier of the tile).
@@ -35,7 +47,7 @@ __global__ void __launch_bounds__(256) pass_kernel(PassArgs p) {
     for (int i = threadIdx.x; i < p.L; i += blockDim.x) ltw[i] = tw[i];
     tw = ltw;
   }
-  run_tile<T>(p, blockIdx.x, bufA, bufB, threadIdx.x, blockDim.x, DevBarrier(), tw);
+  run_tile<T>(p, xcd_tile(blockIdx.x, gridDim.x), bufA, bufB, threadIdx.x, blockDim.x, DevBarrier(), tw);
 }
 
 // One radix pass of the power-of-two path for this thread: read half,
@@ -43,7 +55,7 @@ __global__ void __launch_bounds__(256) pass_kernel(PassArgs p) {
 template <typename T> struct DevPassRun {
   int tid, nth;
   template <class PS> __device__ void operator()(PS &ps) const {
-    cx<T> v[P3E_FAST_E];
+    cx<T> v[PS::SLOTS];
     if (tid < nth) ps.read(tid, nth, v);
     __syncthreads();
     if (tid < nth) ps.write(tid, nth, v);
@@ -51,7 +63,7 @@ template <typename T> struct DevPassRun {
   }
 };
 
-// A plain phase of the power-of-two path for this thread, then a barrier.
+// A plain phase of a tile for this thread, then a barrier.
 struct DevEach {
   int tid;
   template <class F> __device__ void operator()(F f) const {
@@ -84,6 +96,24 @@ __global__ void __launch_bounds__(P3E_FAST_MAX_THREADS) pass_kernel_pow2(PassArg
   DevPassRun<T> run{tid, nth};
   DevEach each{tid};
   fast_tile<T, M>(p, blockIdx.x, buf, nth, run, each);
+}
+
+// Single-buffer mixed-radix path: one padded LDS buffer, then the twiddles
+// when they fit.
+template <typename T>
+__global__ void __launch_bounds__(512) pass_kernel_mid(PassArgs p) {
+  extern __shared__ unsigned char smem_raw[];
+  cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
+  const cx<T> *tw = reinterpret_cast<const cx<T> *>(p.tw);
+  const int tid = threadIdx.x, nth = blockDim.x;
+  if (p.tw_in_lds) {
+    cx<T> *ltw = buf + (size_t)p.B * p.lpad;
+    for (int i = tid; i < p.L; i += nth) ltw[i] = tw[i];
+    tw = ltw;
+  }
+  DevPassRun<T> run{tid, nth};
+  DevEach each{tid};
+  mid_tile<T>(p, xcd_tile(blockIdx.x, gridDim.x), buf, nth, run, each, tw);
 }
 
 // Batch step: a grid-stride loop over a work list of (pass, tile) entries
@@ -126,6 +156,10 @@ __global__ void __launch_bounds__(P3E_STEP_THREADS)
         case 2048: step_pow2<T, 2048>(sp, wi.tile, bufA); break;
         default: step_pow2<T, 4096>(sp, wi.tile, bufA); break;
       }
+    } else if (sp.mid) {
+      DevPassRun<T> run{(int)threadIdx.x, sp.nthreads};
+      DevEach each{(int)threadIdx.x};
+      mid_tile<T>(sp, wi.tile, bufA, sp.nthreads, run, each, reinterpret_cast<const cx<T> *>(sp.tw));
     } else {
       cx<T> *bufB = bufA + (size_t)sp.B * sp.lpad;
       run_tile<T>(sp, wi.tile, bufA, bufB, threadIdx.x, blockDim.x, DevBarrier(),
@@ -189,6 +223,16 @@ extern "C" int p3e_launch_pass(const p3e::PassArgs *a, int prec, size_t lds_byte
     if (threads < 1 || threads > P3E_FAST_MAX_THREADS) return (int)hipErrorInvalidValue;
     return prec == 8 ? p3e::launch_pow2_any<double>(a, lds_bytes, threads, nblocks, st)
                      : p3e::launch_pow2_any<float>(a, lds_bytes, threads, nblocks, st);
+  }
+  if (a->mid) {
+    if (threads < 1 || threads > 512) return (int)hipErrorInvalidValue;
+    const void *fn = prec == 8 ? (const void *)p3e::pass_kernel_mid<double> : (const void *)p3e::pass_kernel_mid<float>;
+    if (lds_bytes > 65536) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    if (prec == 8)
+      hipLaunchKernelGGL(p3e::pass_kernel_mid<double>, dim3(nblocks), dim3(threads), lds_bytes, st, *a);
+    else
+      hipLaunchKernelGGL(p3e::pass_kernel_mid<float>, dim3(nblocks), dim3(threads), lds_bytes, st, *a);
+    return (int)hipGetLastError();
   }
   if (prec == 8) {
     if (lds_bytes > 65536)

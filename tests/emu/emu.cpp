@@ -11,6 +11,7 @@
 #include "../../p3dfft.3_amd/csrc/host/pass_plan.h"
 #include "../../p3dfft.3_amd/csrc/kernels/fft_core.h"
 #include "../../p3dfft.3_amd/csrc/kernels/fft_fast.h"
+#include "../../p3dfft.3_amd/csrc/kernels/fft_mid.h"
 
 using namespace p3e;
 
@@ -33,8 +34,9 @@ template <typename T> struct HostPassRun {
   int nth;
   std::vector<cx<T>> v;
   template <class PS> void operator()(PS &ps) {
-    for (int tid = 0; tid < nth; tid++) ps.read(tid, nth, &v[(size_t)tid * P3E_FAST_E]);
-    for (int tid = 0; tid < nth; tid++) ps.write(tid, nth, &v[(size_t)tid * P3E_FAST_E]);
+    v.assign((size_t)nth * PS::SLOTS, cx<T>{(T)1e30, (T)-1e30});
+    for (int tid = 0; tid < nth; tid++) ps.read(tid, nth, &v[(size_t)tid * PS::SLOTS]);
+    for (int tid = 0; tid < nth; tid++) ps.write(tid, nth, &v[(size_t)tid * PS::SLOTS]);
   }
 };
 
@@ -70,6 +72,16 @@ template <typename T> static int run_pass(const PassDesc &pd, const void *in, vo
   a.post = post.empty() ? nullptr : post.data();
   a.wN = wn.empty() ? nullptr : wn.data();
   a.deriv = deriv;
+  if (a.mid) {
+    HostPassRun<T> run{pd.threads, {}};
+    HostEach each{pd.threads};
+    std::vector<cx<T>> buf((size_t)a.B * a.lpad);
+    for (unsigned int t = 0; t < pd.nblocks; t++) {
+      for (auto &x : buf) x = cx<T>{(T)1e30, (T)-1e30};
+      mid_tile<T>(a, t, buf.data(), pd.threads, run, each, reinterpret_cast<const cx<T> *>(a.tw));
+    }
+    return 0;
+  }
   if (a.fast) {
     switch (a.M) {
       case 16: return run_pow2<T, 16>(pd, a);
