@@ -130,8 +130,8 @@ int tune_cache_size() {
 static std::string tune_key(int prec, const PassDesc &pd) {
   const PassArgs &a = pd.args;
   char k[256];
-  snprintf(k, sizeof(k), "gfx950:p%d:c%d:k%d:N%d:W%d:L%d:o%d%d:e%d", prec, a.cls, a.r2r_kind, a.N, a.W, a.L,
-           a.load_order, a.store_order, a.ne);
+  snprintf(k, sizeof(k), "gfx950:p%d:c%d:k%d:N%d:W%d:L%d:o%d%d:e%d:m%d", prec, a.cls, a.r2r_kind, a.N, a.W, a.L,
+           a.load_order, a.store_order, a.ne, a.mid);
   return k;
 }
 
@@ -172,7 +172,7 @@ static float time_candidate(int prec, PassDesc &pd, void *din, void *dout, PassA
 
 static void tune_pass(int prec, PassDesc &pd, hipStream_t st) {
   PassArgs &a = pd.args;
-  if (a.fast || a.mid || a.cls == CLS_EMPTY) return;
+  if (a.fast || a.cls == CLS_EMPTY) return;
   if (!tables_ready(pd)) return;  // never launch a pass whose tables are not on the device
   load_cache();
   const std::string key = tune_key(prec, pd);
@@ -196,7 +196,7 @@ static void tune_pass(int prec, PassDesc &pd, hipStream_t st) {
     std::vector<std::vector<int>> rads = radix_candidates(a.L, 4);
     TuneChoice best{a.B, std::vector<int>(a.radix, a.radix + a.nrad)};
     float best_ms = time_candidate(prec, pd, din, dout, d_tab, d_work, maxwork, st);
-    for (int B : {4, 8, 16, 32}) {
+    for (int B : {2, 4, 8, 16, 32}) {
       for (const auto &r : rads) {
         PassDesc t = pd;
         if (!retune_pass(prec, t, B, r.data(), (int)r.size())) continue;

@@ -593,17 +593,26 @@ static bool radix_supported(int r) {
 
 bool retune_pass(int prec, PassDesc &ps, int B, const int *rad, int nrad) {
   PassArgs &a = ps.args;
-  if (a.fast || a.mid || a.cls == CLS_EMPTY || B < 1 || (B & (B - 1)) || nrad < 0 ||
-      nrad > P3E_MAX_RADIX_PASSES)
+  if (a.fast || a.cls == CLS_EMPTY || B < 1 || (B & (B - 1)) || nrad < 0 || nrad > P3E_MAX_RADIX_PASSES)
     return false;
   long prod = 1;
   for (int i = 0; i < nrad; i++) {
     if (!radix_supported(rad[i])) return false;
+    if (a.mid && rad[i] != 2 && rad[i] != 3 && rad[i] != 4 && rad[i] != 5 && rad[i] != 8 && rad[i] != 16)
+      return false;
     prod *= rad[i];
   }
   if (prod != a.L) return false;
-  const size_t lds = 2 * (size_t)a.lpad * 2 * (size_t)prec * B;
+  // passes planned for the batch kernel: no tables in LDS; a single-buffer
+  // pass keeps at most 16 elements per thread of the batch workgroup
+  const size_t lds = (a.mid ? 1 : 2) * (size_t)a.lpad * 2 * (size_t)prec * B;
   if (lds > LDS_MAX) return false;
+  if (a.mid) {
+    if ((long)B * a.L > 16L * P3E_STEP_THREADS) return false;
+    const int th = std::max(64, ((int)(((long)B * a.L + 15) / 16) + 63) & ~63);
+    ps.threads = th;
+    a.nthreads = th;
+  }
   for (int i = 0; i < nrad; i++) a.radix[i] = rad[i];
   a.nrad = nrad;
   set_radix_divs(a);

@@ -73,8 +73,9 @@ bool plan_chain(int prec, const OpSpec ops[3], const int order[3], const Layout 
 // integers.
 bool plan_poisson(int prec, const int dims[3], const int mo_real[3], const int mo_spec[3], PassChain &fwd,
                   PassChain &bwd, int batch_threads = 0);
-// Re-tile a generic-path pass with B lines per tile and the given radix
-// sequence (the autotuner's knobs).  Returns false if the choice is invalid.
+// Re-tile a generic-path or single-buffer mixed-radix pass planned for the
+// batch kernel with B lines per tile and the given radix sequence (the
+// autotuner's knobs).  Returns false if the choice is invalid.
 bool retune_pass(int prec, PassDesc &ps, int B, const int *rad, int nrad);
 // True when every constant table the planner built for this pass (twiddles,
 // chirps, pre/post factors, R2C/C2R split factors) has a device address in

@@ -4,6 +4,7 @@
 // one workgroup at a time with a single thread and a no-op barrier.  This
 // checks the kernel's indexing, radix passes, extension rules and layout
 // changes without a GPU.
+#include <stdio.h>
 #include <string.h>
 
 #include <vector>
@@ -157,6 +158,51 @@ int emu_exec(int prec, const int *cls, const int *sign, const int *kind, const i
   }
   run_chain(prec, ch, in, out, idir);
   return 0;
+}
+
+// As emu_exec in batch mode, with every pass that accepts it re-tiled by
+// the autotuner's retune_pass to B lines and the radix sequence that is the
+// first of radix_candidates(L) in descending order (`pick` selects another
+// candidate).  Returns the number of re-tiled passes, or -1.
+int emu_exec_retuned(int prec, const int *cls, const int *sign, const int *kind, const int *idt, const int *odt,
+                     const int *order, const int *in_dims, const int *in_mo, int in_dt, const int *out_dims,
+                     const int *out_mo, int out_dt, const void *in, void *out, int B, int pick, char *desc,
+                     int desc_len) {
+  OpSpec ops[3];
+  for (int a = 0; a < 3; a++) ops[a] = OpSpec{cls[a], sign[a], kind[a], idt[a], odt[a]};
+  Layout li, lo;
+  for (int i = 0; i < 3; i++) {
+    li.dims[i] = in_dims[i];
+    li.mo[i] = in_mo[i];
+    lo.dims[i] = out_dims[i];
+    lo.mo[i] = out_mo[i];
+  }
+  li.dt = in_dt;
+  lo.dt = out_dt;
+  PassChain ch;
+  if (!plan_chain(prec, ops, order, li, lo, ch, P3E_STEP_THREADS)) return -1;
+  int n = 0;
+  std::string d;
+  for (PassDesc &pd : ch.passes) {
+    std::vector<std::vector<int>> c = radix_candidates(pd.args.L, 4);
+    if (c.empty()) continue;
+    const std::vector<int> &r = c[(size_t)pick % c.size()];
+    PassDesc t = pd;
+    if (retune_pass(prec, t, B, r.data(), (int)r.size())) {
+      pd = t;
+      n++;
+      char buf[96];
+      snprintf(buf, sizeof(buf), "[%s B=%d nrad=%d th=%d] ", pd.args.mid ? "mid" : "gen", pd.args.B, pd.args.nrad,
+               pd.threads);
+      d += buf;
+    }
+  }
+  if (desc && desc_len > 0) {
+    strncpy(desc, d.c_str(), desc_len - 1);
+    desc[desc_len - 1] = 0;
+  }
+  run_chain(prec, ch, in, out, -1);
+  return n;
 }
 
 // Periodic Poisson solve of a real grid through the product's Poisson plan

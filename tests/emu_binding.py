@@ -117,3 +117,25 @@ def poisson(dims, rhs, prec, mo_real=(0, 1, 2), mo_spec=(1, 2, 0), batch_threads
     if rc != 0:
         raise RuntimeError(lib().emu_error().decode())
     return sol, spec, desc.value.decode()
+
+
+def exec3d_retuned(types, in_dims, mo1, out_dims, mo2, data, prec, B, pick=0):
+    """Batch-mode plan with every pass re-tiled by the autotuner's
+    retune_pass (B lines, radix candidate `pick`); returns (output, number
+    of re-tiled passes, description)."""
+    cls, sign, kind, idt, odt = ops_for(types)
+    order = orc.find_order(types, mo1, mo2)
+    in_dt = orc.type_info(types[order[0]]).dt1
+    out_dt = orc.type_info(types[order[2]]).dt2
+    rdt = np.float64 if prec == 8 else np.float32
+    cdt = np.complex128 if prec == 8 else np.complex64
+    data = np.ascontiguousarray(data, dtype=cdt if in_dt == 2 else rdt)
+    out = np.zeros(orc.storage_shape(out_dims, mo2), dtype=cdt if out_dt == 2 else rdt)
+    desc = ctypes.create_string_buffer(512)
+    n = lib().emu_exec_retuned(prec, I3(*cls), I3(*sign), I3(*kind), I3(*idt), I3(*odt), I3(*order),
+                               I3(*in_dims), I3(*mo1), in_dt, I3(*out_dims), I3(*mo2), out_dt,
+                               data.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p),
+                               int(B), int(pick), desc, 512)
+    if n < 0:
+        raise RuntimeError(lib().emu_error().decode())
+    return out, n, desc.value.decode()

@@ -177,3 +177,17 @@ def test_launch_guard_requires_device_tables(types, dims):
     od = C.out_dims(types, dims)
     n, before, after = emu.tables_ready_counts(types, dims, [0, 1, 2], od, [0, 1, 2], orc.type_info(types[0]).prec)
     assert n == 3 and before == 3 and after == 0
+
+
+@pytest.mark.parametrize("B,pick", [(2, 0), (4, 1), (8, 2), (16, 3)])
+@pytest.mark.parametrize("dims", [[48, 60, 36], [56, 40, 24]])
+def test_autotuner_retiled_passes_keep_parity(dims, B, pick):
+    """Every re-tiling the autotuner may choose (tile size, radix order) for
+    single-buffer (48, 60, 36, 40, 24) and generic (56 = 8 * 7) passes
+    gives the oracle's result."""
+    t = [C.CF_D] * 3
+    x = C.random_input(t, dims, [0, 1, 2], 11)
+    got, n, desc = emu.exec3d_retuned(t, dims, [0, 1, 2], dims, [0, 1, 2], x, 8, B, pick)
+    ref = orc.exec3d(t, dims, [0, 1, 2], [0, 1, 2], x)
+    assert n >= 1, desc
+    assert C.rel_l2(got, ref) < 1e-12, desc

@@ -164,9 +164,10 @@ def test_batch_graph_replays_match_eager():
 
 
 def test_batch_tuned_vs_oracle():
-    """Autotuned tiles and radix orders (non-power-of-two sizes) keep parity,
-    and a second plan reads the same choices from the cache."""
-    dims = [[48, 96, 120], [120, 48, 96], [96, 120, 48], [60, 36, 100]]
+    """Autotuned tiles and radix orders (non-power-of-two sizes: single-buffer
+    passes, and a generic-path pass for the radix 7 in 56) keep parity, and a
+    second plan reads the same choices from the cache."""
+    dims = [[48, 96, 120], [120, 48, 96], [96, 120, 48], [60, 36, 100], [56, 48, 40]]
     log("case: batch tuned c2c fp32")
     xs = batch_inputs(dims, 7, 4)
     got, desc = run_batch([C.CF_S] * 3, dims, dims, [0, 1, 2], [0, 1, 2], xs, True, 4, tune=True)
