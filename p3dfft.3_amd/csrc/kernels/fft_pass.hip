@@ -76,8 +76,10 @@ struct DevEach {
 // the tile, followed in LDS by the twiddle table (M entries) and, for
 // R2C/C2R, the split factors (M + 1 entries): the radix passes and the
 // split/merge read them from LDS instead of issuing global loads (which
-// would compete with the tile's stream for the memory pipeline).  The copy
-// is complete before the barrier that ends the load phase.
+// would compete with the tile's stream for the memory pipeline).  The
+// twiddles are first read after the barrier that ends the load phase; the
+// split factors are read by the C2R load phase itself, so R2C/C2R passes
+// wait at a barrier after copying them.
 template <typename T, int M>
 __global__ void __launch_bounds__(P3E_FAST_MAX_THREADS) pass_kernel_pow2(PassArgs p0) {
   extern __shared__ unsigned char smem_raw[];
@@ -88,8 +90,12 @@ __global__ void __launch_bounds__(P3E_FAST_MAX_THREADS) pass_kernel_pow2(PassArg
   const cx<T> *gwn = reinterpret_cast<const cx<T> *>(p0.wN);
   const int tid = threadIdx.x, nth = blockDim.x;
   for (int i = tid; i < M; i += nth) ltw[i] = gtw[i];
-  if (p0.cls != CLS_C2C)
+  if (p0.cls != CLS_C2C) {
     for (int i = tid; i <= M; i += nth) lwn[i] = gwn[i];
+    // the C2R load phase merges (X_k, X_{M-k}) with the split factors, so
+    // they must be in LDS before any thread's load phase
+    __syncthreads();
+  }
   PassArgs p = p0;
   p.tw = ltw;
   p.wN = lwn;
