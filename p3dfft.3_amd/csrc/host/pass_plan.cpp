@@ -70,7 +70,7 @@ static bool factor(int n, std::vector<int> &rad) {
     else twos.push_back(2);
   }
   std::vector<int> odd;
-  for (int q : {3, 5, 7, 11, 13, 17, 19, 23, 29, 31}) {
+  for (int q : {3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47}) {
     while (m % q == 0) {
       odd.push_back(q);
       m /= q;
@@ -158,18 +158,23 @@ void set_fast_path(bool enable) { g_fast_enabled = enable; }
 
 static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
+static bool radix_supported(int r);
+
 // Switch an already described generic pass to the single-buffer mixed-radix
-// path (fft_mid.h) when every radix is 2, 3, 4, 5, 8 or 16 and no Bluestein
-// is needed.  Tile: B lines with B * L <= 16 threads' worth of elements, as
+// path (fft_mid.h) when every radix is 2, 3, 4, 5, 8, 16 or an odd prime up
+// to 47 and no Bluestein is needed.  Tile: B lines with B * L <= 16 threads' worth of elements, as
 // large as LDS and the workgroup size (512 threads, or the batch kernel's)
 // allow; the twiddles go to LDS after the lines when they fit.
 static void setup_mid(int prec, const OpSpec &op, PassDesc &ps, int batch_threads) {
   PassArgs &a = ps.args;
   if (a.bluestein || op.cls == CLS_EMPTY || a.nrad < 1) return;
+  bool primes = false;
   for (int i = 0; i < a.nrad; i++) {
-    const int r = a.radix[i];
-    if (r != 2 && r != 3 && r != 4 && r != 5 && r != 8 && r != 16) return;
+    if (!radix_supported(a.radix[i])) return;
+    if (a.radix[i] >= 7) primes = true;
   }
+  // the batch kernel runs the single-buffer path for radices 2..16 only
+  if (primes && batch_threads > 0) return;
   const int L = a.L;
   const long maxth = batch_threads > 0 ? batch_threads : 512;
   const size_t esz = 2 * (size_t)prec;
@@ -183,7 +188,7 @@ static void setup_mid(int prec, const OpSpec &op, PassDesc &ps, int batch_thread
   int th = (int)(((long)B * L + 15) / 16);
   th = std::max(64, (th + 63) & ~63);
   if (th > maxth) return;
-  a.mid = 1;
+  a.mid = primes ? 2 : 1;
   a.tw_in_lds = twl > 0;
   a.B = B;
   a.logB = 0;
@@ -585,7 +590,7 @@ bool plan_chain(int prec, const OpSpec ops[3], const int order[3], const Layout 
 // ------------------------------------------------------------------ tuning knobs
 static bool radix_supported(int r) {
   if (r == 2 || r == 3 || r == 4 || r == 5 || r == 8 || r == 16) return true;
-  if (r < 7 || r > 31 || r % 2 == 0) return false;
+  if (r < 7 || r > 47 || r % 2 == 0) return false;
   for (int q = 3; q * q <= r; q += 2)
     if (r % q == 0) return false;
   return true;
@@ -598,8 +603,7 @@ bool retune_pass(int prec, PassDesc &ps, int B, const int *rad, int nrad) {
   long prod = 1;
   for (int i = 0; i < nrad; i++) {
     if (!radix_supported(rad[i])) return false;
-    if (a.mid && rad[i] != 2 && rad[i] != 3 && rad[i] != 4 && rad[i] != 5 && rad[i] != 8 && rad[i] != 16)
-      return false;
+    if (a.mid && rad[i] >= 7) return false;  // the batch kernel's single-buffer path: radices 2..16
     prod *= rad[i];
   }
   if (prod != a.L) return false;
@@ -683,7 +687,7 @@ static void factorizations(int n, int maxr, std::vector<int> &cur, std::vector<s
     return;
   }
   if (cur.size() >= 6) return;
-  static const int R[] = {16, 8, 5, 4, 3, 2, 7, 11, 13, 17, 19, 23, 29, 31};
+  static const int R[] = {16, 8, 5, 4, 3, 2, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47};
   for (int r : R) {
     if (r > maxr || n % r) continue;
     cur.push_back(r);
