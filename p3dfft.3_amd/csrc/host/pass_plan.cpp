@@ -161,20 +161,17 @@ static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 static bool radix_supported(int r);
 
 // Switch an already described generic pass to the single-buffer mixed-radix
-// path (fft_mid.h) when every radix is 2, 3, 4, 5, 8, 16 or an odd prime up
-// to 47 and no Bluestein is needed.  Tile: B lines with B * L <= 16 threads' worth of elements, as
+// path (fft_mid.h) when every radix is 2, 3, 4, 5, 8 or 16 and no Bluestein
+// is needed (odd prime radices 7..47 stay on the two-buffer generic path).  Tile: B lines with B * L <= 16 threads' worth of elements, as
 // large as LDS and the workgroup size (512 threads, or the batch kernel's)
 // allow; the twiddles go to LDS after the lines when they fit.
 static void setup_mid(int prec, const OpSpec &op, PassDesc &ps, int batch_threads) {
   PassArgs &a = ps.args;
   if (a.bluestein || op.cls == CLS_EMPTY || a.nrad < 1) return;
-  bool primes = false;
   for (int i = 0; i < a.nrad; i++) {
-    if (!radix_supported(a.radix[i])) return;
-    if (a.radix[i] >= 7) primes = true;
+    const int r = a.radix[i];
+    if (r != 2 && r != 3 && r != 4 && r != 5 && r != 8 && r != 16) return;
   }
-  // the batch kernel runs the single-buffer path for radices 2..16 only
-  if (primes && batch_threads > 0) return;
   const int L = a.L;
   const long maxth = batch_threads > 0 ? batch_threads : 512;
   const size_t esz = 2 * (size_t)prec;
@@ -188,7 +185,7 @@ static void setup_mid(int prec, const OpSpec &op, PassDesc &ps, int batch_thread
   int th = (int)(((long)B * L + 15) / 16);
   th = std::max(64, (th + 63) & ~63);
   if (th > maxth) return;
-  a.mid = primes ? 2 : 1;
+  a.mid = 1;
   a.tw_in_lds = twl > 0;
   a.B = B;
   a.logB = 0;
@@ -603,7 +600,8 @@ bool retune_pass(int prec, PassDesc &ps, int B, const int *rad, int nrad) {
   long prod = 1;
   for (int i = 0; i < nrad; i++) {
     if (!radix_supported(rad[i])) return false;
-    if (a.mid && rad[i] >= 7) return false;  // the batch kernel's single-buffer path: radices 2..16
+    // the single-buffer path: radices 2, 3, 4, 5, 8, 16
+    if (a.mid && rad[i] >= 7 && rad[i] != 8 && rad[i] != 16) return false;
     prod *= rad[i];
   }
   if (prod != a.L) return false;

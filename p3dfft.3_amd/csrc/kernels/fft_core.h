@@ -197,31 +197,34 @@ P3E_HD P3E_INL void stockham_pass(const cx<T> *src, cx<T> *dst, int L, int lpad,
   }
 }
 
-// Generic odd radix (7..31): outputs computed one at a time from LDS.
+// Odd prime radix R (7..47): a direct DFT, one thread per output.  Output o
+// of the pass is (line, butterfly j = (o mod L) / R, q = o mod R):
+// X_q = sum_r x_r W^{rq}, x_r = input r of butterfly j times its Stockham
+// twiddle, W = e^{s 2 pi i / R} = the table root at L/R.  Consecutive threads
+// share a butterfly, so its inputs are LDS broadcasts.
 template <typename T>
 P3E_HD void stockham_pass_generic(const cx<T> *src, cx<T> *dst, int L, int lpad, int nb, int Ns, int R, int s,
                                   const cx<T> *tw, int tid, int nth, const FastDiv &dnbf, const FastDiv &dns) {
   const int nbf = L / R;
   const int tstride = L / (Ns * R);
   const int wstride = L / R;
-  const int total = nb * nbf;
-  for (int t = tid; t < total; t += nth) {
-    const int line = (int)fdiv((uint32_t)t, dnbf);
-    const int j = t - line * nbf;
-    const cx<T> *sl = src + line * lpad;
-    cx<T> *dl = dst + line * lpad;
+  const int total = nb * L;
+  for (int o = tid; o < total; o += nth) {
+    const int b = o / R, q = o - b * R;
+    const int line = (int)fdiv((uint32_t)b, dnbf);
+    const int j = b - line * nbf;
     const int k = j - (int)fdiv((uint32_t)j, dns) * Ns;
-    const int base = (j - k) * R + k;
-    for (int q = 0; q < R; q++) {
-      cx<T> acc = mk((T)0, (T)0);
-      for (int r = 0; r < R; r++) {
-        cx<T> x = sl[P3E_PAD(j + r * nbf)];
-        if (Ns > 1 && r > 0) x = mul(x, twid(tw, k * r * tstride, s));
-        int e = (r * q) % R;
-        acc = add(acc, mul(x, twid(tw, e * wstride, s)));
-      }
-      dl[P3E_PAD(base + q * Ns)] = acc;
+    const cx<T> *sl = src + line * lpad;
+    cx<T> acc = sl[P3E_PAD(j)];
+    int e = 0;
+    for (int r = 1; r < R; r++) {
+      cx<T> x = sl[P3E_PAD(j + r * nbf)];
+      if (Ns > 1) x = mul(x, twid(tw, k * r * tstride, s));
+      e += q;
+      if (e >= R) e -= R;
+      acc = add(acc, mul(x, twid(tw, e * wstride, s)));
     }
+    dst[line * lpad + P3E_PAD((j - k) * R + k + q * Ns)] = acc;
   }
 }
 

@@ -1,9 +1,8 @@
 // Single-buffer mixed-radix path of the fused reorder + 1D transform pass.
 //
-// For work lengths L whose radices are all in {2, 3, 4, 5, 8, 16} or odd
-// primes up to 47 (no Bluestein): the same load, scatter, output rule and
-// store as the generic path (fft_core.h), but the radix passes run in place
-// on ONE padded LDS
+// For work lengths L whose radices are all in {2, 3, 4, 5, 8, 16} (no
+// Bluestein): the same load, scatter, output rule and store as the generic
+// path (fft_core.h), but the radix passes run in place on ONE padded LDS
 // buffer, as in the power-of-two path: every thread reads the butterflies
 // it owns into registers, barrier, twiddles + butterflies + writes to the
 // Stockham positions, barrier.  Half the LDS of the two-buffer generic path
@@ -55,40 +54,20 @@ template <typename T, int R> struct MPass {
 #pragma unroll
           for (int r = 1; r < R; r++) w[r] = mul(w[r], twid(tw, k * r * tstride, s));
         }
+        bfly<T, R>(w, s);
         cx<T> *dl = buf + line * lpad;
         const int base = (j - k) * R + k;
-        if constexpr (R == 2 || R == 3 || R == 4 || R == 5 || R == 8 || R == 16) {
-          bfly<T, R>(w, s);
 #pragma unroll
-          for (int r = 0; r < R; r++) dl[P3E_PAD(base + r * Ns)] = w[r];
-        } else {
-          // odd prime R: direct DFT from registers, X_q = sum_r w_r W^{rq}
-          // with W = e^{s 2 pi i / R} = the root at L/R in the table; each
-          // output goes to LDS as soon as it is summed
-          const int wstride = L / R;
-          for (int q = 0; q < R; q++) {
-            cx<T> acc = w[0];
-            int e = 0;
-#pragma unroll
-            for (int r = 1; r < R; r++) {
-              e += q;
-              if (e >= R) e -= R;
-              acc = add(acc, mul(w[r], twid(tw, e * wstride, s)));
-            }
-            dl[P3E_PAD(base + q * Ns)] = acc;
-          }
-        }
+        for (int r = 0; r < R; r++) dl[P3E_PAD(base + r * Ns)] = w[r];
       }
     }
   }
 };
 
 // FFT of length L on nb lines in place; `run` executes one pass (read half,
-// barrier, write half, barrier).  Primes: also the odd prime radices 7..47
-// (a kernel's register budget is that of its largest radix, so the small
-// radices have kernels of their own).
-template <typename T, bool Primes, typename Run>
-P3E_HD void mid_fft(Run &run, cx<T> *buf, const PassArgs &p, int nb, int s, const cx<T> *tw) {
+// barrier, write half, barrier).
+template <typename T, typename Run>
+P3E_HD P3E_INL void mid_fft(Run &run, cx<T> *buf, const PassArgs &p, int nb, int s, const cx<T> *tw) {
   int Ns = 1;
   for (int i = 0; i < p.nrad; i++) {
     const int R = p.radix[i];
@@ -99,21 +78,7 @@ P3E_HD void mid_fft(Run &run, cx<T> *buf, const PassArgs &p, int nb, int s, cons
       case 4: { MPass<T, 4> ps{buf, p.L, p.lpad, nb, Ns, s, tw, dn, ds}; run(ps); } break;
       case 5: { MPass<T, 5> ps{buf, p.L, p.lpad, nb, Ns, s, tw, dn, ds}; run(ps); } break;
       case 8: { MPass<T, 8> ps{buf, p.L, p.lpad, nb, Ns, s, tw, dn, ds}; run(ps); } break;
-      case 16: { MPass<T, 16> ps{buf, p.L, p.lpad, nb, Ns, s, tw, dn, ds}; run(ps); } break;
-      default: if constexpr (Primes) switch (R) {
-      case 7: { MPass<T, 7> ps{buf, p.L, p.lpad, nb, Ns, s, tw, dn, ds}; run(ps); } break;
-      case 11: { MPass<T, 11> ps{buf, p.L, p.lpad, nb, Ns, s, tw, dn, ds}; run(ps); } break;
-      case 13: { MPass<T, 13> ps{buf, p.L, p.lpad, nb, Ns, s, tw, dn, ds}; run(ps); } break;
-      case 17: { MPass<T, 17> ps{buf, p.L, p.lpad, nb, Ns, s, tw, dn, ds}; run(ps); } break;
-      case 19: { MPass<T, 19> ps{buf, p.L, p.lpad, nb, Ns, s, tw, dn, ds}; run(ps); } break;
-      case 23: { MPass<T, 23> ps{buf, p.L, p.lpad, nb, Ns, s, tw, dn, ds}; run(ps); } break;
-      case 29: { MPass<T, 29> ps{buf, p.L, p.lpad, nb, Ns, s, tw, dn, ds}; run(ps); } break;
-      case 31: { MPass<T, 31> ps{buf, p.L, p.lpad, nb, Ns, s, tw, dn, ds}; run(ps); } break;
-      case 37: { MPass<T, 37> ps{buf, p.L, p.lpad, nb, Ns, s, tw, dn, ds}; run(ps); } break;
-      case 41: { MPass<T, 41> ps{buf, p.L, p.lpad, nb, Ns, s, tw, dn, ds}; run(ps); } break;
-      case 43: { MPass<T, 43> ps{buf, p.L, p.lpad, nb, Ns, s, tw, dn, ds}; run(ps); } break;
-      default: { MPass<T, 47> ps{buf, p.L, p.lpad, nb, Ns, s, tw, dn, ds}; run(ps); } break;
-      }
+      default: { MPass<T, 16> ps{buf, p.L, p.lpad, nb, Ns, s, tw, dn, ds}; run(ps); } break;
     }
     Ns *= R;
   }
@@ -122,8 +87,8 @@ P3E_HD void mid_fft(Run &run, cx<T> *buf, const PassArgs &p, int nb, int s, cons
 // Whole tile.  `each(f)` runs phase f for the calling thread and ends with a
 // barrier (device), or runs it for every thread in turn (host build).  Only
 // threads tid < nth work; the others take part in the barriers.
-template <typename T, bool Primes, typename Run, typename Each>
-P3E_HD void mid_tile(const PassArgs &p, uint32_t tile, cx<T> *buf, int nth, Run &run, Each &each,
+template <typename T, typename Run, typename Each>
+P3E_HD P3E_INL void mid_tile(const PassArgs &p, uint32_t tile, cx<T> *buf, int nth, Run &run, Each &each,
                      const cx<T> *tw) {
   uint32_t f;
   int e0, nb;
@@ -135,7 +100,7 @@ P3E_HD void mid_tile(const PassArgs &p, uint32_t tile, cx<T> *buf, int nth, Run 
   each([&](int tid) {
     if (tid < nth) generic_load<T>(p, f, e0, nb, buf, tid, nth);
   });
-  if (p.cls != CLS_EMPTY) mid_fft<T, Primes>(run, buf, p, nb, p.sign, tw);
+  if (p.cls != CLS_EMPTY) mid_fft<T>(run, buf, p, nb, p.sign, tw);
   each([&](int tid) {
     if (tid < nth) generic_store<T>(p, f, e0, nb, buf, tid, nth);
   });

@@ -105,8 +105,8 @@ __global__ void __launch_bounds__(P3E_FAST_MAX_THREADS) pass_kernel_pow2(PassArg
 }
 
 // Single-buffer mixed-radix path: one padded LDS buffer, then the twiddles
-// when they fit.  Primes: the kernel for lengths with odd prime radices.
-template <typename T, bool Primes>
+// when they fit.
+template <typename T>
 __global__ void __launch_bounds__(512) pass_kernel_mid(PassArgs p) {
   extern __shared__ unsigned char smem_raw[];
   cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
@@ -119,7 +119,7 @@ __global__ void __launch_bounds__(512) pass_kernel_mid(PassArgs p) {
   }
   DevPassRun<T> run{tid, nth};
   DevEach each{tid};
-  mid_tile<T, Primes>(p, xcd_tile(blockIdx.x, gridDim.x), buf, nth, run, each, tw);
+  mid_tile<T>(p, xcd_tile(blockIdx.x, gridDim.x), buf, nth, run, each, tw);
 }
 
 // Batch step: a grid-stride loop over a work list of (pass, tile) entries
@@ -165,7 +165,7 @@ __global__ void __launch_bounds__(P3E_STEP_THREADS)
     } else if (sp.mid) {
       DevPassRun<T> run{(int)threadIdx.x, sp.nthreads};
       DevEach each{(int)threadIdx.x};
-      mid_tile<T, false>(sp, wi.tile, bufA, sp.nthreads, run, each, reinterpret_cast<const cx<T> *>(sp.tw));
+      mid_tile<T>(sp, wi.tile, bufA, sp.nthreads, run, each, reinterpret_cast<const cx<T> *>(sp.tw));
     } else {
       cx<T> *bufB = bufA + (size_t)sp.B * sp.lpad;
       run_tile<T>(sp, wi.tile, bufA, bufB, threadIdx.x, blockDim.x, DevBarrier(),
@@ -233,9 +233,7 @@ extern "C" int p3e_launch_pass(const p3e::PassArgs *a, int prec, size_t lds_byte
   if (a->mid) {
     if (threads < 1 || threads > 512) return (int)hipErrorInvalidValue;
     typedef void (*MidFn)(p3e::PassArgs);
-    const bool primes = a->mid == 2;
-    MidFn fn = prec == 8 ? (primes ? p3e::pass_kernel_mid<double, true> : p3e::pass_kernel_mid<double, false>)
-                         : (primes ? p3e::pass_kernel_mid<float, true> : p3e::pass_kernel_mid<float, false>);
+    MidFn fn = prec == 8 ? p3e::pass_kernel_mid<double> : p3e::pass_kernel_mid<float>;
     if (lds_bytes > 65536)
       (void)hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
     hipLaunchKernelGGL(fn, dim3(nblocks), dim3(threads), lds_bytes, st, *a);

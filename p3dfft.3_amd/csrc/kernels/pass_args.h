@@ -76,8 +76,7 @@ struct PassArgs {
   int nthreads;      // threads that work on a tile (a batch launch may have more)
   FastDiv div_pairs; // C2R: division by M/2 + 1, the pairs (X_k, X_{M-k}) per line
   int tw_in_lds;     // generic path: the standalone kernel copies tw to LDS
-  int mid;           // single-buffer mixed-radix path (fft_mid.h): 1 radices
-                     // 2..16 only, 2 also odd primes 7..47
+  int mid;           // single-buffer mixed-radix path (fft_mid.h)
   // generic path, radix pass i: division by L / radix[i] (butterflies per
   // line) and by the pass's input stride Ns = radix[0] ... radix[i-1]
   FastDiv div_nbf[P3E_MAX_RADIX_PASSES], div_ns[P3E_MAX_RADIX_PASSES];

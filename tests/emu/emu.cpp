@@ -79,7 +79,7 @@ template <typename T> static int run_pass(const PassDesc &pd, const void *in, vo
     std::vector<cx<T>> buf((size_t)a.B * a.lpad);
     for (unsigned int t = 0; t < pd.nblocks; t++) {
       for (auto &x : buf) x = cx<T>{(T)1e30, (T)-1e30};
-      mid_tile<T, true>(a, t, buf.data(), pd.threads, run, each, reinterpret_cast<const cx<T> *>(a.tw));
+      mid_tile<T>(a, t, buf.data(), pd.threads, run, each, reinterpret_cast<const cx<T> *>(a.tw));
     }
     return 0;
   }
