@@ -70,7 +70,7 @@ static bool factor(int n, std::vector<int> &rad) {
     else twos.push_back(2);
   }
   std::vector<int> odd;
-  for (int q : {3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47}) {
+  for (int q : {3, 5, 7, 11, 13, 17, 19, 23, 29, 31}) {
     while (m % q == 0) {
       odd.push_back(q);
       m /= q;
@@ -161,13 +161,14 @@ static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 static bool radix_supported(int r);
 
 // Switch an already described generic pass to the single-buffer mixed-radix
-// path (fft_mid.h) when every radix is 2, 3, 4, 5, 8 or 16 and no Bluestein
-// is needed (odd prime radices 7..47 stay on the two-buffer generic path).  Tile: B lines with B * L <= 16 threads' worth of elements, as
+// path (fft_mid.h) when every radix of the work length (the Bluestein length
+// for a prime above 31) is 2, 3, 4, 5, 8 or 16; odd prime radices 7..31 stay
+// on the two-buffer generic path.  Tile: B lines with B * L <= 16 threads' worth of elements, as
 // large as LDS and the workgroup size (512 threads, or the batch kernel's)
 // allow; the twiddles go to LDS after the lines when they fit.
 static void setup_mid(int prec, const OpSpec &op, PassDesc &ps, int batch_threads) {
   PassArgs &a = ps.args;
-  if (a.bluestein || op.cls == CLS_EMPTY || a.nrad < 1) return;
+  if (op.cls == CLS_EMPTY || a.nrad < 1) return;
   for (int i = 0; i < a.nrad; i++) {
     const int r = a.radix[i];
     if (r != 2 && r != 3 && r != 4 && r != 5 && r != 8 && r != 16) return;
@@ -575,7 +576,7 @@ bool plan_chain(int prec, const OpSpec ops[3], const int order[3], const Layout 
     chain.pass_of_axis[axes[i]] = (int)chain.passes.size();
     snprintf(buf, sizeof(buf), "%s%s[axis %d, N=%d%s, mo (%d%d%d)->(%d%d%d), B=%d]", i ? " | " : "",
              cls_name(ops[axes[i]].cls), axes[i], ps.args.N,
-             ps.args.bluestein ? ", bluestein" : (ps.args.fast ? ", pow2" : (ps.args.mid ? ", mid" : "")),
+             ps.args.bluestein ? (ps.args.mid ? ", bluestein mid" : ", bluestein") : (ps.args.fast ? ", pow2" : (ps.args.mid ? ", mid" : "")),
              lay[i].mo[0], lay[i].mo[1], lay[i].mo[2], lay[i + 1].mo[0], lay[i + 1].mo[1], lay[i + 1].mo[2],
              ps.args.B);
     chain.desc += buf;
@@ -587,7 +588,7 @@ bool plan_chain(int prec, const OpSpec ops[3], const int order[3], const Layout 
 // ------------------------------------------------------------------ tuning knobs
 static bool radix_supported(int r) {
   if (r == 2 || r == 3 || r == 4 || r == 5 || r == 8 || r == 16) return true;
-  if (r < 7 || r > 47 || r % 2 == 0) return false;
+  if (r < 7 || r > 31 || r % 2 == 0) return false;
   for (int q = 3; q * q <= r; q += 2)
     if (r % q == 0) return false;
   return true;
@@ -685,7 +686,7 @@ static void factorizations(int n, int maxr, std::vector<int> &cur, std::vector<s
     return;
   }
   if (cur.size() >= 6) return;
-  static const int R[] = {16, 8, 5, 4, 3, 2, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47};
+  static const int R[] = {16, 8, 5, 4, 3, 2, 7, 11, 13, 17, 19, 23, 29, 31};
   for (int r : R) {
     if (r > maxr || n % r) continue;
     cur.push_back(r);

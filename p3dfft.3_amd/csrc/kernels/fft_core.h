@@ -197,7 +197,7 @@ P3E_HD P3E_INL void stockham_pass(const cx<T> *src, cx<T> *dst, int L, int lpad,
   }
 }
 
-// Odd prime radix R (7..47): a direct DFT, one thread per output.  Output o
+// Odd prime radix R (7..31): a direct DFT, one thread per output.  Output o
 // of the pass is (line, butterfly j = (o mod L) / R, q = o mod R):
 // X_q = sum_r x_r W^{rq}, x_r = input r of butterfly j times its Stockham
 // twiddle, W = e^{s 2 pi i / R} = the table root at L/R.  Consecutive threads

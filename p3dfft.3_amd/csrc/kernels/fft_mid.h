@@ -1,7 +1,8 @@
 // Single-buffer mixed-radix path of the fused reorder + 1D transform pass.
 //
-// For work lengths L whose radices are all in {2, 3, 4, 5, 8, 16} (no
-// Bluestein): the same load, scatter, output rule and store as the generic
+// For work lengths L whose radices are all in {2, 3, 4, 5, 8, 16} (also the
+// power-of-two Bluestein lengths): the same load, scatter, output rule and
+// store as the generic
 // path (fft_core.h), but the radix passes run in place on ONE padded LDS
 // buffer, as in the power-of-two path: every thread reads the butterflies
 // it owns into registers, barrier, twiddles + butterflies + writes to the
@@ -100,7 +101,26 @@ P3E_HD P3E_INL void mid_tile(const PassArgs &p, uint32_t tile, cx<T> *buf, int n
   each([&](int tid) {
     if (tid < nth) generic_load<T>(p, f, e0, nb, buf, tid, nth);
   });
-  if (p.cls != CLS_EMPTY) mid_fft<T>(run, buf, p, nb, p.sign, tw);
+  if (p.cls != CLS_EMPTY) {
+    if (!p.bluestein) {
+      mid_fft<T>(run, buf, p, nb, p.sign, tw);
+    } else {
+      // Bluestein: forward FFT of the chirped sequence, multiply by the
+      // spectrum of the conjugate chirp, inverse FFT (the store applies the
+      // output chirp)
+      mid_fft<T>(run, buf, p, nb, -1, tw);
+      const cx<T> *chirp_hat = reinterpret_cast<const cx<T> *>(p.chirp_hat);
+      each([&](int tid) {
+        if (tid < nth)
+          for (int i = tid; i < nb * p.L; i += nth) {
+            const int line = i / p.L;
+            const int m = i - line * p.L;
+            buf[line * p.lpad + P3E_PAD(m)] = mul(buf[line * p.lpad + P3E_PAD(m)], chirp_hat[m]);
+          }
+      });
+      mid_fft<T>(run, buf, p, nb, +1, tw);
+    }
+  }
   each([&](int tid) {
     if (tid < nth) generic_store<T>(p, f, e0, nb, buf, tid, nth);
   });

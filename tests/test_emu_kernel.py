@@ -195,14 +195,13 @@ def test_autotuner_retiled_passes_keep_parity(dims, B, pick):
 
 @pytest.mark.parametrize("prec", [8, 4])
 @pytest.mark.parametrize("types,dims", [
-    ([C.R2C_D, C.CF_D, C.CF_D], [344, 6, 4]),       # 8 * 43
-    ([C.CF_D] * 3, [37, 41, 47]),                   # single primes 37, 41, 47
-    ([C.CF_D] * 3, [296, 5, 3]),                    # 8 * 37
+    ([C.R2C_D, C.CF_D, C.CF_D], [232, 14, 26]),     # 8 * 29, 2 * 7, 2 * 13
+    ([C.CF_D] * 3, [31, 17, 19]),                   # single primes
     ([C.R2C_D, C.CF_D, C.CF_D], [176, 22, 26]),     # radices 11 and 13
-    ([C.r2r_id("DCT2", True, 8)] * 3, [43, 19, 23]),
+    ([C.r2r_id("DCT2", True, 8)] * 3, [23, 19, 29]),
 ])
-def test_odd_prime_radices_single_buffer(types, dims, prec):
-    """Odd prime radices 7..47 run as direct DFTs parallel over outputs (no
+def test_odd_prime_radices_direct(types, dims, prec):
+    """Odd prime radices 7..31 run as direct DFTs parallel over outputs (no
     Bluestein) and match the oracle."""
     t = [x - 1 if prec == 4 else x for x in types]
     x = C.random_input(types, dims, [0, 1, 2], 5)
@@ -211,4 +210,22 @@ def test_odd_prime_radices_single_buffer(types, dims, prec):
     got, desc = emu.exec3d(t, dims, [0, 1, 2], od, [0, 1, 2], xin, prec)
     ref = orc.exec3d(types, dims, [0, 1, 2], [0, 1, 2], x)
     assert "bluestein" not in desc, desc
+    assert C.rel_l2(got, ref) < (1e-12 if prec == 8 else 1e-5), desc
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+@pytest.mark.parametrize("types,dims", [
+    ([C.R2C_D, C.CF_D, C.CF_D], [344, 6, 4]),       # 8 * 43
+    ([C.CF_D] * 3, [37, 41, 47]),
+    ([C.r2r_id("DST1", False, 8)] * 3, [36, 7, 5]),  # W = 74 = 2 * 37
+])
+def test_bluestein_single_buffer(types, dims, prec):
+    """Primes above 31 take Bluestein on the single-buffer path."""
+    t = [x - 1 if prec == 4 else x for x in types]
+    x = C.random_input(types, dims, [0, 1, 2], 6)
+    od = C.out_dims(types, dims)
+    xin = x.astype(np.complex64 if np.iscomplexobj(x) else np.float32) if prec == 4 else x
+    got, desc = emu.exec3d(t, dims, [0, 1, 2], od, [0, 1, 2], xin, prec)
+    ref = orc.exec3d(types, dims, [0, 1, 2], [0, 1, 2], x)
+    assert "bluestein" in desc, desc
     assert C.rel_l2(got, ref) < (1e-12 if prec == 8 else 1e-5), desc
