@@ -39,7 +39,7 @@ struct PoissonPlan {
   size_t real_bytes;
 };
 
-static const long long DEDICATED_POINTS = 8LL << 20;
+static const long long DEDICATED_POINTS = 1LL << 20;
 
 struct PoissonRequest {
   long long id;

@@ -159,7 +159,7 @@ def test_service_rejects_oversized_request():
 
 
 def test_service_large_grid_dedicated_launches():
-    """A grid of at least 8M points runs its passes as their own launches,
+    """A grid of at least 2^20 points runs its passes as their own launches,
     mixed with small requests in the same steps: parity with the oracle, and
     the same bits alone and in the mix."""
     big = [256, 256, 128]
