@@ -202,8 +202,8 @@ static void service_loop(PoissonServer *s) {
   std::unique_lock<std::mutex> lk(s->mu);
   while (!s->failed) {
     // completed steps
-    for (int k = 0; k < 2; k++) {
-      PoissonServer::Step &b = s->steps[(s->cur + k) % 2];
+    for (int k = 0; k < SERVE_DEPTH; k++) {
+      PoissonServer::Step &b = s->steps[(s->cur + k) % SERVE_DEPTH];
       if (b.busy && hipEventQuery(b.done) == hipSuccess) retire(s, b);
     }
     // admission: arrival order, a slot and working memory
@@ -220,9 +220,9 @@ static void service_loop(PoissonServer *s) {
     }
     if (s->active.empty()) {
       PoissonServer::Step *oldest = nullptr;
-      for (int k = 0; k < 2; k++)
-        if (s->steps[(s->cur + k) % 2].busy) {
-          oldest = &s->steps[(s->cur + k) % 2];
+      for (int k = 0; k < SERVE_DEPTH; k++)
+        if (s->steps[(s->cur + k) % SERVE_DEPTH].busy) {
+          oldest = &s->steps[(s->cur + k) % SERVE_DEPTH];
           break;
         }
       if (oldest) {
@@ -263,7 +263,7 @@ static void service_loop(PoissonServer *s) {
       else still.push_back(r);
     }
     s->active.swap(still);
-    s->cur ^= 1;
+    s->cur = (s->cur + 1) % SERVE_DEPTH;
   }
 }
 

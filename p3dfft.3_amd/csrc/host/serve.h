@@ -2,12 +2,13 @@
 // continuously batched at pass granularity on one GPU.
 //
 // Each admitted request advances one pass per step; a step is one launch of
-// the step kernel over the current pass of every request in flight, so
-// requests that arrive while others run join at their first pass in the next
-// step and their work shares launches with the others'.  Requests are
-// admitted in arrival order when a slot and their working memory (two
-// buffers of the largest pass output, from one pool allocated up front) are
-// free.  Plans and constant tables stay resident per grid size.  Two steps
+// the step kernel over the current pass of every small request in flight
+// (plus one launch of the per-pass kernels for each large one), so requests
+// that arrive while others run join at their first pass in the next step
+// and their work shares launches with the others'.  Requests are admitted in
+// arrival order when a slot and their working memory (two buffers of the
+// largest pass output, from one pool allocated up front) are free.  Plans
+// and constant tables stay resident per grid size.  Up to SERVE_DEPTH steps
 // may be in flight; a request's memory returns to the pool when the step
 // with its last pass has completed.
 #pragma once
@@ -40,6 +41,10 @@ struct PoissonPlan {
 };
 
 static const long long DEDICATED_POINTS = 1LL << 20;
+// Steps in flight on the device: the service thread builds the next steps
+// while earlier ones run, so the GPU does not wait for the host between
+// steps.
+static const int SERVE_DEPTH = 4;
 
 struct PoissonRequest {
   long long id;
@@ -68,7 +73,7 @@ struct PoissonServer {
     hipEvent_t done;
     bool busy;
     std::vector<PoissonRequest *> finishing;
-  } steps[2];
+  } steps[SERVE_DEPTH];
   int cur;
   std::mutex mu;
   std::condition_variable cv, done_cv;
