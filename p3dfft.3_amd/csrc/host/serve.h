@@ -41,10 +41,9 @@ struct PoissonPlan {
 };
 
 static const long long DEDICATED_POINTS = 1LL << 20;
-// Steps in flight on the device: the service thread builds the next steps
-// while earlier ones run, so the GPU does not wait for the host between
-// steps.
-static const int SERVE_DEPTH = 4;
+// Steps in flight on the device: the service thread builds the next step
+// while the earlier one runs.
+static const int SERVE_DEPTH = 2;
 
 struct PoissonRequest {
   long long id;
