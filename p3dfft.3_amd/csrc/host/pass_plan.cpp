@@ -50,9 +50,9 @@ static int stride1_axis(const int mo[3]) {
 }
 
 // Radix plan for length n: 8s (with one 16 or 4 absorbing the remainder) for
-// the power-of-two part, then odd primes up to 31, largest first.  Returns
+// the power-of-two part, then odd primes up to maxp, largest first.  Returns
 // false if a larger prime factor remains (the pass then uses Bluestein).
-static bool factor(int n, std::vector<int> &rad) {
+static bool factor(int n, std::vector<int> &rad, int maxp = 31) {
   rad.clear();
   int m = n, p2 = 0;
   while (m % 2 == 0) {
@@ -70,7 +70,8 @@ static bool factor(int n, std::vector<int> &rad) {
     else twos.push_back(2);
   }
   std::vector<int> odd;
-  for (int q : {3, 5, 7, 11, 13, 17, 19, 23, 29, 31}) {
+  for (int q : {3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47}) {
+    if (q > maxp) break;
     while (m % q == 0) {
       odd.push_back(q);
       m /= q;
@@ -160,40 +161,69 @@ static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
 static bool radix_supported(int r);
 
+static bool mid_base_radix(int r) { return r == 2 || r == 3 || r == 4 || r == 5 || r == 8 || r == 16; }
+
+// Largest odd prime radix of the single-buffer kernels: 47 in FP32; 29 in
+// FP64, where the registers of a larger butterfly (two doubles per complex
+// value) exceed the 256 VGPRs of a wave and would spill.
+static int mid_max_prime(int prec) { return prec == 8 ? 29 : 47; }
+
+// The odd prime radix of a radix list that the single-buffer path takes: 0
+// if every radix is 2, 3, 4, 5, 8 or 16, the prime if there is one distinct
+// other radix and it is a prime from 7 to mid_max_prime(prec), -1 otherwise.
+static int mid_prime_of(const int *rad, int nrad, int prec) {
+  int p = 0;
+  for (int i = 0; i < nrad; i++) {
+    const int r = rad[i];
+    if (mid_base_radix(r)) continue;
+    if (r < 7 || r > mid_max_prime(prec) || (p != 0 && r != p)) return -1;
+    for (int q = 3; q * q <= r; q += 2)
+      if (r % q == 0) return -1;
+    p = r;
+  }
+  return p;
+}
+
 // Switch an already described generic pass to the single-buffer mixed-radix
 // path (fft_mid.h) when every radix of the work length (the Bluestein length
-// for a prime above 31) is 2, 3, 4, 5, 8 or 16; odd prime radices 7..31 stay
-// on the two-buffer generic path.  Tile: B lines with B * L <= 16 threads' worth of elements, as
+// for a large prime) is 2, 3, 4, 5, 8 or 16, plus — for a standalone pass
+// kernel, not the batch kernel — at most one distinct odd prime 7..47 (its
+// own kernel instantiation).  Other passes stay on the two-buffer generic
+// path.  Tile: B lines with B * L <= 16 threads' worth of elements, as
 // large as LDS and the workgroup size (512 threads, or the batch kernel's)
 // allow; the twiddles go to LDS after the lines when they fit.
 static void setup_mid(int prec, const OpSpec &op, PassDesc &ps, int batch_threads) {
   PassArgs &a = ps.args;
   if (op.cls == CLS_EMPTY || a.nrad < 1) return;
-  for (int i = 0; i < a.nrad; i++) {
-    const int r = a.radix[i];
-    if (r != 2 && r != 3 && r != 4 && r != 5 && r != 8 && r != 16) return;
-  }
+  const int prime = mid_prime_of(a.radix, a.nrad, prec);
+  if (prime < 0 || (prime > 0 && batch_threads > 0)) return;
   const int L = a.L;
   const long maxth = batch_threads > 0 ? batch_threads : 512;
   const size_t esz = 2 * (size_t)prec;
   const size_t per_line = (size_t)a.lpad * esz;
   size_t twl = batch_threads > 0 ? 0 : (size_t)L * esz;
   if (per_line + twl > LDS_MAX) twl = 0;
+  // R2C / C2R: two real lines per LDS line (fft_mid.h, pair_load)
+  const bool pair = op.cls == CLS_R2C || op.cls == CLS_C2R;
+  auto lines = [&](int b) { return pair ? (b + 1) / 2 : b; };
   int B = 1;
-  const int Bmax = std::min(32, next_pow2(a.ne));
-  while (B * 2 <= Bmax && per_line * (B * 2) + twl <= LDS_BUDGET && (long)B * 2 * L <= 16 * maxth) B *= 2;
-  if (per_line * B + twl > LDS_MAX || (long)B * L > 16 * maxth) return;
-  int th = (int)(((long)B * L + 15) / 16);
+  const int Bmax = std::min(pair ? 64 : 32, next_pow2(a.ne));
+  while (B * 2 <= Bmax && per_line * lines(B * 2) + twl <= LDS_BUDGET && (long)lines(B * 2) * L <= 16 * maxth)
+    B *= 2;
+  if (per_line * lines(B) + twl > LDS_MAX || (long)lines(B) * L > 16 * maxth) return;
+  int th = (int)(((long)lines(B) * L + 15) / 16);
   th = std::max(64, (th + 63) & ~63);
   if (th > maxth) return;
   a.mid = 1;
+  a.mid_prime = prime;
+  a.pair = pair;
   a.tw_in_lds = twl > 0;
   a.B = B;
   a.logB = 0;
   while ((1 << a.logB) < B) a.logB++;
   a.ntile_e = (a.ne + B - 1) / B;
   a.div_ntile = make_fastdiv((uint32_t)a.ntile_e);
-  ps.lds_bytes = per_line * B + twl;
+  ps.lds_bytes = per_line * lines(B) + twl;
   ps.threads = th;
   a.nthreads = th;
   ps.nblocks = (unsigned int)((unsigned long long)a.ntile_e * (unsigned long long)a.nf);
@@ -242,6 +272,8 @@ static void setup_fast(int prec, const OpSpec &op, PassDesc &ps, int batch_threa
   if (lds(B) > LDS_MAX || (long)M * B / P3E_FAST_E > maxth) return;
   a.fast = 1;
   a.mid = 0;
+  a.mid_prime = 0;
+  a.pair = 0;
   a.tw_in_lds = 0;
   a.M = M;
   a.logM = 0;
@@ -346,10 +378,23 @@ bool describe_pass(int prec, int axis, const OpSpec &op, const Layout &lin, cons
   a.sign = sign;
   std::vector<int> rad;
   int L = W;
-  if (op.cls != CLS_EMPTY && !factor(W, rad)) {
-    a.bluestein = 1;
-    L = next_pow2(2 * W - 1);
-    factor(L, rad);
+  if (op.cls != CLS_EMPTY) {
+    // A standalone FP32 pass takes odd primes up to 47 as radices when the
+    // single-buffer path can run the whole length (one distinct such prime,
+    // at most 16 elements per thread of 512, one padded line in LDS);
+    // otherwise primes up to 31 run as direct DFTs on the generic path, and
+    // a length with a larger prime factor uses Bluestein.
+    bool ok = false;
+    if (batch_threads == 0 && factor(W, rad, mid_max_prime(prec)) &&
+        mid_prime_of(rad.data(), (int)rad.size(), prec) > 31 &&
+        W <= 16 * 512 && (size_t)generic_pitch(W) * 2 * (size_t)prec <= LDS_MAX)
+      ok = true;
+    if (!ok) ok = factor(W, rad);
+    if (!ok) {
+      a.bluestein = 1;
+      L = next_pow2(2 * W - 1);
+      factor(L, rad);
+    }
   }
   a.L = L;
   a.nrad = (int)rad.size();
@@ -574,9 +619,15 @@ bool plan_chain(int prec, const OpSpec ops[3], const int order[3], const Layout 
       return false;
     }
     chain.pass_of_axis[axes[i]] = (int)chain.passes.size();
+    char kind[40];
+    if (ps.args.mid_prime) snprintf(kind, sizeof(kind), ", mid p%d", ps.args.mid_prime);
+    else
+      snprintf(kind, sizeof(kind), "%s",
+               ps.args.bluestein ? (ps.args.mid ? ", bluestein mid" : ", bluestein")
+                                 : (ps.args.fast ? ", pow2" : (ps.args.mid ? ", mid" : "")));
+    if (ps.args.pair) strncat(kind, " pair", sizeof(kind) - strlen(kind) - 1);
     snprintf(buf, sizeof(buf), "%s%s[axis %d, N=%d%s, mo (%d%d%d)->(%d%d%d), B=%d]", i ? " | " : "",
-             cls_name(ops[axes[i]].cls), axes[i], ps.args.N,
-             ps.args.bluestein ? (ps.args.mid ? ", bluestein mid" : ", bluestein") : (ps.args.fast ? ", pow2" : (ps.args.mid ? ", mid" : "")),
+             cls_name(ops[axes[i]].cls), axes[i], ps.args.N, kind,
              lay[i].mo[0], lay[i].mo[1], lay[i].mo[2], lay[i + 1].mo[0], lay[i + 1].mo[1], lay[i + 1].mo[2],
              ps.args.B);
     chain.desc += buf;
@@ -596,23 +647,25 @@ static bool radix_supported(int r) {
 
 bool retune_pass(int prec, PassDesc &ps, int B, const int *rad, int nrad) {
   PassArgs &a = ps.args;
-  if (a.fast || a.cls == CLS_EMPTY || B < 1 || (B & (B - 1)) || nrad < 0 || nrad > P3E_MAX_RADIX_PASSES)
+  if (a.fast || a.mid_prime || a.cls == CLS_EMPTY || B < 1 || (B & (B - 1)) || nrad < 0 ||
+      nrad > P3E_MAX_RADIX_PASSES)
     return false;
   long prod = 1;
   for (int i = 0; i < nrad; i++) {
     if (!radix_supported(rad[i])) return false;
-    // the single-buffer path: radices 2, 3, 4, 5, 8, 16
-    if (a.mid && rad[i] >= 7 && rad[i] != 8 && rad[i] != 16) return false;
+    // the single-buffer path of the batch kernel: radices 2, 3, 4, 5, 8, 16
+    if (a.mid && !mid_base_radix(rad[i])) return false;
     prod *= rad[i];
   }
   if (prod != a.L) return false;
   // passes planned for the batch kernel: no tables in LDS; a single-buffer
   // pass keeps at most 16 elements per thread of the batch workgroup
-  const size_t lds = (a.mid ? 1 : 2) * (size_t)a.lpad * 2 * (size_t)prec * B;
+  const int nl = a.pair ? (B + 1) / 2 : B;  // LDS lines of a tile
+  const size_t lds = (a.mid ? 1 : 2) * (size_t)a.lpad * 2 * (size_t)prec * nl;
   if (lds > LDS_MAX) return false;
   if (a.mid) {
-    if ((long)B * a.L > 16L * P3E_STEP_THREADS) return false;
-    const int th = std::max(64, ((int)(((long)B * a.L + 15) / 16) + 63) & ~63);
+    if ((long)nl * a.L > 16L * P3E_STEP_THREADS) return false;
+    const int th = std::max(64, ((int)(((long)nl * a.L + 15) / 16) + 63) & ~63);
     ps.threads = th;
     a.nthreads = th;
   }

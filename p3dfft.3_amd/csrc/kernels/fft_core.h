@@ -155,6 +155,89 @@ template <typename T> P3E_HD P3E_INL void bf16(cx<T> *v, int s) {
   }
 }
 
+// cos and sin of 2 pi m / P for m = 0 .. P-1, evaluated at compile time:
+// the angle is reduced to an octant exactly in integers (8m = o P + r), the
+// remaining angle (pi/4) r/P or (pi/4)(P-r)/P lies in [0, pi/4], and sin/cos
+// there are Taylor series summed to x^27 (the next term is below 1e-30).
+constexpr double cx_pi4 = 0.78539816339744830961566084581987572;
+constexpr void taylor_sincos(double x, double &sn, double &cs) {
+  double t = x, s = 0.0;
+  for (int k = 1; k < 28; k += 2) {
+    s += t;
+    t *= -x * x / ((k + 1) * (k + 2));
+  }
+  double u = 1.0, c = 0.0;
+  for (int k = 0; k < 28; k += 2) {
+    c += u;
+    u *= -x * x / ((k + 1) * (k + 2));
+  }
+  sn = s;
+  cs = c;
+}
+template <int P> struct PrimeRoots {
+  double c[P], s[P];
+  constexpr PrimeRoots() : c(), s() {
+    for (int m = 0; m < P; m++) {
+      const int o = (8 * m) / P, r = 8 * m - o * P;
+      const bool odd = o & 1;
+      double sx = 0.0, cx = 0.0;
+      taylor_sincos(cx_pi4 * (double)(odd ? P - r : r) / (double)P, sx, cx);
+      // angle = o pi/4 + x (o even) or (o+1) pi/4 - x (o odd)
+      double cc = 0.0, ss = 0.0;
+      switch (o) {
+        case 0: cc = cx; ss = sx; break;
+        case 1: cc = sx; ss = cx; break;
+        case 2: cc = -sx; ss = cx; break;
+        case 3: cc = -cx; ss = sx; break;
+        case 4: cc = -cx; ss = -sx; break;
+        case 5: cc = -sx; ss = -cx; break;
+        case 6: cc = sx; ss = -cx; break;
+        default: cc = cx; ss = -sx; break;
+      }
+      c[m] = cc;
+      s[m] = ss;
+    }
+  }
+};
+template <int P> constexpr PrimeRoots<P> prime_roots{};
+
+// Odd prime P: with a_j = v_j + v_{P-j}, b_j = v_j - v_{P-j} (j = 1..H,
+// H = (P-1)/2) and theta = 2 pi j k / P,
+//   X_k     = v_0 + sum_j a_j cos(theta) + s i sum_j b_j sin(theta)
+//   X_{P-k} = v_0 + sum_j a_j cos(theta) - s i sum_j b_j sin(theta)
+// so H^2 real-by-complex products for each of the two sums.  put(q, X_q)
+// receives each output as soon as it is formed (the caller stores it), so
+// the outputs need no registers of their own.
+template <typename T, int P, typename Put> P3E_HD P3E_INL void bf_prime(const cx<T> *v, int s, Put put) {
+  constexpr int H = (P - 1) / 2;
+  cx<T> a[H], b[H];
+  const cx<T> x0 = v[0];
+  cx<T> y0 = x0;
+#pragma unroll
+  for (int j = 0; j < H; j++) {
+    a[j] = add(v[j + 1], v[P - 1 - j]);
+    b[j] = sub(v[j + 1], v[P - 1 - j]);
+    y0 = add(y0, a[j]);
+  }
+  put(0, y0);
+#pragma unroll
+  for (int k = 1; k <= H; k++) {
+    cx<T> A = x0, Bs = mk((T)0, (T)0);
+#pragma unroll
+    for (int j = 1; j <= H; j++) {
+      const int m = (j * k) % P;
+      const T c = (T)prime_roots<P>.c[m], sn = (T)prime_roots<P>.s[m];
+      A = mk(A.x + a[j - 1].x * c, A.y + a[j - 1].y * c);
+      Bs = mk(Bs.x + b[j - 1].x * sn, Bs.y + b[j - 1].y * sn);
+    }
+    const cx<T> r = rot(Bs, s);
+    put(k, add(A, r));
+    put(P - k, sub(A, r));
+  }
+}
+
+template <int R> constexpr bool is_base_radix = R == 2 || R == 3 || R == 4 || R == 5 || R == 8 || R == 16;
+
 template <typename T, int R> P3E_HD P3E_INL void bfly(cx<T> *v, int s) {
   if constexpr (R == 2) bf2(v);
   else if constexpr (R == 3) bf3(v, s);
@@ -162,6 +245,7 @@ template <typename T, int R> P3E_HD P3E_INL void bfly(cx<T> *v, int s) {
   else if constexpr (R == 5) bf5(v, s);
   else if constexpr (R == 8) bf8(v, s);
   else if constexpr (R == 16) bf16(v, s);
+  else bf_prime<T, R>(v, s, [v](int q, cx<T> x) { v[q] = x; });
 }
 
 template <typename T> P3E_HD P3E_INL cx<T> twid(const cx<T> *tw, int idx, int s) {

@@ -1,15 +1,18 @@
 // Single-buffer mixed-radix path of the fused reorder + 1D transform pass.
 //
-// For work lengths L whose radices are all in {2, 3, 4, 5, 8, 16} (also the
-// power-of-two Bluestein lengths): the same load, scatter, output rule and
-// store as the generic
-// path (fft_core.h), but the radix passes run in place on ONE padded LDS
+// For work lengths L whose radices are all in {2, 3, 4, 5, 8, 16} plus at
+// most one odd prime P in 7..47 (also the power-of-two Bluestein lengths):
+// the same load, scatter, output rule and store as the generic path
+// (fft_core.h), but the radix passes run in place on ONE padded LDS
 // buffer, as in the power-of-two path: every thread reads the butterflies
 // it owns into registers, barrier, twiddles + butterflies + writes to the
 // Stockham positions, barrier.  Half the LDS of the two-buffer generic path
 // per line, so twice the lines per tile (longer contiguous runs on a
 // strided axis).  The planner sizes the workgroup so that every thread owns
-// at most 16 elements of the tile (nb * L <= 16 * nth).
+// at most 16 elements of the tile (nb * L <= 16 * nth).  The prime P is a
+// template parameter (PassArgs::mid_prime selects the instantiation), so
+// the registers of its butterfly are only allocated in the kernels that
+// need it.
 // Shared by the HIP kernels and the host build used by the CPU tests.
 #pragma once
 #include "fft_core.h"
@@ -55,11 +58,15 @@ template <typename T, int R> struct MPass {
 #pragma unroll
           for (int r = 1; r < R; r++) w[r] = mul(w[r], twid(tw, k * r * tstride, s));
         }
-        bfly<T, R>(w, s);
         cx<T> *dl = buf + line * lpad;
         const int base = (j - k) * R + k;
+        if constexpr (is_base_radix<R>) {
+          bfly<T, R>(w, s);
 #pragma unroll
-        for (int r = 0; r < R; r++) dl[P3E_PAD(base + r * Ns)] = w[r];
+          for (int r = 0; r < R; r++) dl[P3E_PAD(base + r * Ns)] = w[r];
+        } else {
+          bf_prime<T, R>(w, s, [&](int q, cx<T> x) { dl[P3E_PAD(base + q * Ns)] = x; });
+        }
       }
     }
   }
@@ -67,7 +74,7 @@ template <typename T, int R> struct MPass {
 
 // FFT of length L on nb lines in place; `run` executes one pass (read half,
 // barrier, write half, barrier).
-template <typename T, typename Run>
+template <typename T, int P, typename Run>
 P3E_HD P3E_INL void mid_fft(Run &run, cx<T> *buf, const PassArgs &p, int nb, int s, const cx<T> *tw) {
   int Ns = 1;
   for (int i = 0; i < p.nrad; i++) {
@@ -79,50 +86,157 @@ P3E_HD P3E_INL void mid_fft(Run &run, cx<T> *buf, const PassArgs &p, int nb, int
       case 4: { MPass<T, 4> ps{buf, p.L, p.lpad, nb, Ns, s, tw, dn, ds}; run(ps); } break;
       case 5: { MPass<T, 5> ps{buf, p.L, p.lpad, nb, Ns, s, tw, dn, ds}; run(ps); } break;
       case 8: { MPass<T, 8> ps{buf, p.L, p.lpad, nb, Ns, s, tw, dn, ds}; run(ps); } break;
-      default: { MPass<T, 16> ps{buf, p.L, p.lpad, nb, Ns, s, tw, dn, ds}; run(ps); } break;
+      case 16: { MPass<T, 16> ps{buf, p.L, p.lpad, nb, Ns, s, tw, dn, ds}; run(ps); } break;
+      default:
+        // the planner admits no other radix to this instantiation
+        if constexpr (P > 0) {
+          MPass<T, P> ps{buf, p.L, p.lpad, nb, Ns, s, tw, dn, ds};
+          run(ps);
+        }
+        break;
     }
     Ns *= R;
+  }
+}
+
+// ---------------------------------------------------------------- paired lines
+// R2C and C2R with p.pair: one complex FFT of length W = N serves two real
+// lines.  LDS line t holds z = x_{2t} + i x_{2t+1} (line 2t+1 absent -> 0).
+// R2C:  X_{2t,k} = (Z_k + conj Z_{N-k}) / 2,  X_{2t+1,k} = (Z_k - conj Z_{N-k}) / (2i).
+// C2R:  Z_k = Y_{2t,k} + i Y_{2t+1,k} over the Hermitian extension of both
+// half spectra; the imaginary parts of the DC and (even N) Nyquist inputs
+// are dropped first, as a real inverse transform ignores them; the outputs
+// are Re z (line 2t) and Im z (line 2t+1).
+
+// Lines of LDS a tile of nb lines occupies.
+P3E_HD P3E_INL int tile_lines(const PassArgs &p, int nb) { return p.pair ? (nb + 1) / 2 : nb; }
+
+template <typename T>
+P3E_HD P3E_INL void pair_load(const PassArgs &p, uint32_t f, int e0, int nb, cx<T> *buf, int tid, int nth) {
+  const cx<T> *chirp = reinterpret_cast<const cx<T> *>(p.chirp);
+  const long long base = (long long)e0 * p.is_e + (long long)f * p.is_f;
+  const int nl = (nb + 1) / 2, N = p.N;
+  const int lbl = p.logB > 0 ? p.logB - 1 : 0;  // B / 2 line pairs per row (1 if B = 1)
+  const int tot = (p.load_order == ORD_LINE ? nl : 1 << lbl) * p.nin;
+  for (int i = tid; i < tot; i += nth) {
+    int n, t;
+    if (p.load_order == ORD_LINE) {
+      t = (int)fdiv((uint32_t)i, p.div_nin);
+      n = i - t * p.nin;
+    } else {
+      t = i & ((1 << lbl) - 1);
+      n = i >> lbl;
+      if (t >= nl) continue;
+    }
+    const bool two = 2 * t + 1 < nb;
+    const long long off = base + (long long)n * p.is_d + (long long)(2 * t) * p.is_e;
+    cx<T> *line = buf + t * p.lpad;
+    if (p.cls == CLS_R2C) {
+      const T *in = reinterpret_cast<const T *>(p.in);
+      put(line, n, mk(in[off], two ? in[off + p.is_e] : (T)0), p, chirp);
+    } else {
+      const cx<T> *in = reinterpret_cast<const cx<T> *>(p.in);
+      cx<T> a = in[off], b = two ? in[off + p.is_e] : mk((T)0, (T)0);
+      if (n == 0 || 2 * n == N) {
+        a.y = (T)0;
+        b.y = (T)0;
+      }
+      put(line, n, mk(a.x - b.y, a.y + b.x), p, chirp);
+      if (n >= 1 && n < (N + 1) / 2) put(line, N - n, mk(a.x + b.y, b.x - a.y), p, chirp);
+    }
+  }
+}
+
+template <typename T>
+P3E_HD P3E_INL void pair_store(const PassArgs &p, uint32_t f, int e0, int nb, const cx<T> *res, int tid,
+                               int nth) {
+  const cx<T> *chirp = reinterpret_cast<const cx<T> *>(p.chirp);
+  const long long base = (long long)e0 * p.os_e + (long long)f * p.os_f;
+  const int N = p.N;
+  const int tot = (p.store_order == ORD_LINE ? nb : p.B) * p.nout;
+  for (int i = tid; i < tot; i += nth) {
+    int k, j;
+    if (p.store_order == ORD_LINE) {
+      j = (int)fdiv((uint32_t)i, p.div_nout);
+      k = i - j * p.nout;
+    } else {
+      j = i & (p.B - 1);
+      k = i >> p.logB;
+      if (j >= nb) continue;
+    }
+    const cx<T> *line = res + (j >> 1) * p.lpad;
+    const long long off = base + (long long)k * p.os_d + (long long)j * p.os_e;
+    cx<T> z = line[P3E_PAD(k)];
+    if (p.bluestein) z = mul(z, chirp[k]);
+    cx<T> y;
+    if (p.cls == CLS_C2R) {
+      y = mk((j & 1) ? z.y : z.x, (T)0);
+    } else {
+      const int km = k ? N - k : 0;
+      cx<T> zm = line[P3E_PAD(km)];
+      if (p.bluestein) zm = mul(zm, chirp[km]);
+      y = (j & 1) ? mk((T)0.5 * (z.y + zm.y), (T)0.5 * (zm.x - z.x))
+                  : mk((T)0.5 * (z.x + zm.x), (T)0.5 * (z.y - zm.y));
+    }
+    if (p.deriv) {
+      T kk;
+      if (k < p.dmid) kk = (T)k;
+      else if (k == p.dmid) kk = (T)0;
+      else kk = (T)(k - p.dg);
+      y = mk(-kk * y.y, kk * y.x);
+    }
+    if (p.spec_op) y = scl(y, poisson_factor<T>(p, k, e0 + j, (int)f));
+    if (p.has_scale) y = scl(y, (T)p.scale);
+    if (p.out_cplx) reinterpret_cast<cx<T> *>(p.out)[off] = y;
+    else reinterpret_cast<T *>(p.out)[off] = y.x;
   }
 }
 
 // Whole tile.  `each(f)` runs phase f for the calling thread and ends with a
 // barrier (device), or runs it for every thread in turn (host build).  Only
 // threads tid < nth work; the others take part in the barriers.
-template <typename T, typename Run, typename Each>
+template <typename T, int P, typename Run, typename Each>
 P3E_HD P3E_INL void mid_tile(const PassArgs &p, uint32_t tile, cx<T> *buf, int nth, Run &run, Each &each,
                      const cx<T> *tw) {
   uint32_t f;
   int e0, nb;
   generic_tile_coords(p, tile, &f, &e0, &nb);
+  const int nl = tile_lines(p, nb);
   if (p.zero_fill)
     each([&](int tid) {
-      if (tid < nth) generic_zero<T>(p, nb, buf, tid, nth);
+      if (tid < nth) generic_zero<T>(p, nl, buf, tid, nth);
     });
   each([&](int tid) {
-    if (tid < nth) generic_load<T>(p, f, e0, nb, buf, tid, nth);
+    if (tid < nth) {
+      if (p.pair) pair_load<T>(p, f, e0, nb, buf, tid, nth);
+      else generic_load<T>(p, f, e0, nb, buf, tid, nth);
+    }
   });
   if (p.cls != CLS_EMPTY) {
     if (!p.bluestein) {
-      mid_fft<T>(run, buf, p, nb, p.sign, tw);
+      mid_fft<T, P>(run, buf, p, nl, p.sign, tw);
     } else {
       // Bluestein: forward FFT of the chirped sequence, multiply by the
       // spectrum of the conjugate chirp, inverse FFT (the store applies the
       // output chirp)
-      mid_fft<T>(run, buf, p, nb, -1, tw);
+      mid_fft<T, P>(run, buf, p, nl, -1, tw);
       const cx<T> *chirp_hat = reinterpret_cast<const cx<T> *>(p.chirp_hat);
       each([&](int tid) {
         if (tid < nth)
-          for (int i = tid; i < nb * p.L; i += nth) {
+          for (int i = tid; i < nl * p.L; i += nth) {
             const int line = i / p.L;
             const int m = i - line * p.L;
             buf[line * p.lpad + P3E_PAD(m)] = mul(buf[line * p.lpad + P3E_PAD(m)], chirp_hat[m]);
           }
       });
-      mid_fft<T>(run, buf, p, nb, +1, tw);
+      mid_fft<T, P>(run, buf, p, nl, +1, tw);
     }
   }
   each([&](int tid) {
-    if (tid < nth) generic_store<T>(p, f, e0, nb, buf, tid, nth);
+    if (tid < nth) {
+      if (p.pair) pair_store<T>(p, f, e0, nb, buf, tid, nth);
+      else generic_store<T>(p, f, e0, nb, buf, tid, nth);
+    }
   });
 }
 

@@ -105,21 +105,21 @@ __global__ void __launch_bounds__(P3E_FAST_MAX_THREADS) pass_kernel_pow2(PassArg
 }
 
 // Single-buffer mixed-radix path: one padded LDS buffer, then the twiddles
-// when they fit.
-template <typename T>
+// when they fit.  P: the odd prime radix of the pass (0: none).
+template <typename T, int P>
 __global__ void __launch_bounds__(512) pass_kernel_mid(PassArgs p) {
   extern __shared__ unsigned char smem_raw[];
   cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
   const cx<T> *tw = reinterpret_cast<const cx<T> *>(p.tw);
   const int tid = threadIdx.x, nth = blockDim.x;
   if (p.tw_in_lds) {
-    cx<T> *ltw = buf + (size_t)p.B * p.lpad;
+    cx<T> *ltw = buf + (size_t)tile_lines(p, p.B) * p.lpad;
     for (int i = tid; i < p.L; i += nth) ltw[i] = tw[i];
     tw = ltw;
   }
   DevPassRun<T> run{tid, nth};
   DevEach each{tid};
-  mid_tile<T>(p, xcd_tile(blockIdx.x, gridDim.x), buf, nth, run, each, tw);
+  mid_tile<T, P>(p, xcd_tile(blockIdx.x, gridDim.x), buf, nth, run, each, tw);
 }
 
 // Batch step: a grid-stride loop over a work list of (pass, tile) entries
@@ -165,7 +165,8 @@ __global__ void __launch_bounds__(P3E_STEP_THREADS)
     } else if (sp.mid) {
       DevPassRun<T> run{(int)threadIdx.x, sp.nthreads};
       DevEach each{(int)threadIdx.x};
-      mid_tile<T>(sp, wi.tile, bufA, sp.nthreads, run, each, reinterpret_cast<const cx<T> *>(sp.tw));
+      // batch plans admit no prime radix to the single-buffer path
+      mid_tile<T, 0>(sp, wi.tile, bufA, sp.nthreads, run, each, reinterpret_cast<const cx<T> *>(sp.tw));
     } else {
       cx<T> *bufB = bufA + (size_t)sp.B * sp.lpad;
       run_tile<T>(sp, wi.tile, bufA, bufB, threadIdx.x, blockDim.x, DevBarrier(),
@@ -195,6 +196,40 @@ static int launch_pow2_any(const PassArgs *a, size_t lds_bytes, int threads, uns
     case 1024: return launch_pow2<T, 1024>(a, lds_bytes, threads, nblocks, st);
     case 2048: return launch_pow2<T, 2048>(a, lds_bytes, threads, nblocks, st);
     case 4096: return launch_pow2<T, 4096>(a, lds_bytes, threads, nblocks, st);
+  }
+  return (int)hipErrorInvalidValue;
+}
+
+template <typename T, int P>
+static int launch_mid(const PassArgs *a, size_t lds_bytes, int threads, unsigned int nblocks, hipStream_t st) {
+  if (lds_bytes > 65536)
+    (void)hipFuncSetAttribute((const void *)pass_kernel_mid<T, P>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds_bytes);
+  hipLaunchKernelGGL((pass_kernel_mid<T, P>), dim3(nblocks), dim3(threads), lds_bytes, st, *a);
+  return (int)hipGetLastError();
+}
+
+template <typename T>
+static int launch_mid_any(const PassArgs *a, size_t lds_bytes, int threads, unsigned int nblocks, hipStream_t st) {
+  switch (a->mid_prime) {
+    case 0: return launch_mid<T, 0>(a, lds_bytes, threads, nblocks, st);
+    case 7: return launch_mid<T, 7>(a, lds_bytes, threads, nblocks, st);
+    case 11: return launch_mid<T, 11>(a, lds_bytes, threads, nblocks, st);
+    case 13: return launch_mid<T, 13>(a, lds_bytes, threads, nblocks, st);
+    case 17: return launch_mid<T, 17>(a, lds_bytes, threads, nblocks, st);
+    case 19: return launch_mid<T, 19>(a, lds_bytes, threads, nblocks, st);
+    case 23: return launch_mid<T, 23>(a, lds_bytes, threads, nblocks, st);
+    case 29: return launch_mid<T, 29>(a, lds_bytes, threads, nblocks, st);
+  }
+  // primes above 29: FP32 only (the planner's limit for FP64)
+  if constexpr (sizeof(T) == 4) {
+    switch (a->mid_prime) {
+      case 31: return launch_mid<T, 31>(a, lds_bytes, threads, nblocks, st);
+      case 37: return launch_mid<T, 37>(a, lds_bytes, threads, nblocks, st);
+      case 41: return launch_mid<T, 41>(a, lds_bytes, threads, nblocks, st);
+      case 43: return launch_mid<T, 43>(a, lds_bytes, threads, nblocks, st);
+      case 47: return launch_mid<T, 47>(a, lds_bytes, threads, nblocks, st);
+    }
   }
   return (int)hipErrorInvalidValue;
 }
@@ -232,12 +267,8 @@ extern "C" int p3e_launch_pass(const p3e::PassArgs *a, int prec, size_t lds_byte
   }
   if (a->mid) {
     if (threads < 1 || threads > 512) return (int)hipErrorInvalidValue;
-    typedef void (*MidFn)(p3e::PassArgs);
-    MidFn fn = prec == 8 ? p3e::pass_kernel_mid<double> : p3e::pass_kernel_mid<float>;
-    if (lds_bytes > 65536)
-      (void)hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
-    hipLaunchKernelGGL(fn, dim3(nblocks), dim3(threads), lds_bytes, st, *a);
-    return (int)hipGetLastError();
+    return prec == 8 ? p3e::launch_mid_any<double>(a, lds_bytes, threads, nblocks, st)
+                     : p3e::launch_mid_any<float>(a, lds_bytes, threads, nblocks, st);
   }
   if (prec == 8) {
     if (lds_bytes > 65536)

@@ -77,6 +77,9 @@ struct PassArgs {
   FastDiv div_pairs; // C2R: division by M/2 + 1, the pairs (X_k, X_{M-k}) per line
   int tw_in_lds;     // generic path: the standalone kernel copies tw to LDS
   int mid;           // single-buffer mixed-radix path (fft_mid.h)
+  int mid_prime;     // its odd prime radix 7..47 (0: none); selects the kernel instantiation
+  int pair;          // R2C/C2R on the single-buffer path: lines 2t, 2t+1 share LDS line t
+                    // as real and imaginary parts (one complex FFT for two real lines)
   // generic path, radix pass i: division by L / radix[i] (butterflies per
   // line) and by the pass's input stride Ns = radix[0] ... radix[i-1]
   FastDiv div_nbf[P3E_MAX_RADIX_PASSES], div_ns[P3E_MAX_RADIX_PASSES];

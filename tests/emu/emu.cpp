@@ -60,6 +60,18 @@ template <typename T, int M> static int run_pow2(const PassDesc &pd, const PassA
   return 0;
 }
 
+// The single-buffer path for one instantiation of its odd prime radix.
+template <typename T, int P> static int run_mid(const PassDesc &pd, const PassArgs &a) {
+  HostPassRun<T> run{pd.threads, {}};
+  HostEach each{pd.threads};
+  std::vector<cx<T>> buf((size_t)a.B * a.lpad);
+  for (unsigned int t = 0; t < pd.nblocks; t++) {
+    for (auto &x : buf) x = cx<T>{(T)1e30, (T)-1e30};
+    mid_tile<T, P>(a, t, buf.data(), pd.threads, run, each, reinterpret_cast<const cx<T> *>(a.tw));
+  }
+  return 0;
+}
+
 template <typename T> static int run_pass(const PassDesc &pd, const void *in, void *out, int deriv) {
   std::vector<T> tw = tab<T>(pd.tw), ch = tab<T>(pd.chirp), chh = tab<T>(pd.chirp_hat), pre = tab<T>(pd.pre),
                  post = tab<T>(pd.post), wn = tab<T>(pd.wN);
@@ -74,14 +86,22 @@ template <typename T> static int run_pass(const PassDesc &pd, const void *in, vo
   a.wN = wn.empty() ? nullptr : wn.data();
   a.deriv = deriv;
   if (a.mid) {
-    HostPassRun<T> run{pd.threads, {}};
-    HostEach each{pd.threads};
-    std::vector<cx<T>> buf((size_t)a.B * a.lpad);
-    for (unsigned int t = 0; t < pd.nblocks; t++) {
-      for (auto &x : buf) x = cx<T>{(T)1e30, (T)-1e30};
-      mid_tile<T>(a, t, buf.data(), pd.threads, run, each, reinterpret_cast<const cx<T> *>(a.tw));
+    switch (a.mid_prime) {
+      case 0: return run_mid<T, 0>(pd, a);
+      case 7: return run_mid<T, 7>(pd, a);
+      case 11: return run_mid<T, 11>(pd, a);
+      case 13: return run_mid<T, 13>(pd, a);
+      case 17: return run_mid<T, 17>(pd, a);
+      case 19: return run_mid<T, 19>(pd, a);
+      case 23: return run_mid<T, 23>(pd, a);
+      case 29: return run_mid<T, 29>(pd, a);
+      case 31: return run_mid<T, 31>(pd, a);
+      case 37: return run_mid<T, 37>(pd, a);
+      case 41: return run_mid<T, 41>(pd, a);
+      case 43: return run_mid<T, 43>(pd, a);
+      case 47: return run_mid<T, 47>(pd, a);
     }
-    return 0;
+    return -1;
   }
   if (a.fast) {
     switch (a.M) {

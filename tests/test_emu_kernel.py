@@ -193,39 +193,102 @@ def test_autotuner_retiled_passes_keep_parity(dims, B, pick):
     assert C.rel_l2(got, ref) < 1e-12, desc
 
 
+def _run_case(types, dims, prec, seed, batch_threads=0):
+    t = [x - 1 if prec == 4 else x for x in types]
+    x = C.random_input(types, dims, [0, 1, 2], seed)
+    od = C.out_dims(types, dims)
+    xin = x.astype(np.complex64 if np.iscomplexobj(x) else np.float32) if prec == 4 else x
+    got, desc = emu.exec3d(t, dims, [0, 1, 2], od, [0, 1, 2], xin, prec, batch_threads=batch_threads)
+    ref = orc.exec3d(types, dims, [0, 1, 2], [0, 1, 2], x)
+    return C.rel_l2(got, ref), desc
+
+
 @pytest.mark.parametrize("prec", [8, 4])
 @pytest.mark.parametrize("types,dims", [
     ([C.R2C_D, C.CF_D, C.CF_D], [232, 14, 26]),     # 8 * 29, 2 * 7, 2 * 13
     ([C.CF_D] * 3, [31, 17, 19]),                   # single primes
     ([C.R2C_D, C.CF_D, C.CF_D], [176, 22, 26]),     # radices 11 and 13
     ([C.r2r_id("DCT2", True, 8)] * 3, [23, 19, 29]),
+    ([C.CF_D] * 3, [49, 121, 21]),                  # 7 * 7, 11 * 11, 3 * 7
 ])
-def test_odd_prime_radices_direct(types, dims, prec):
-    """Odd prime radices 7..31 run as direct DFTs parallel over outputs (no
-    Bluestein) and match the oracle."""
-    t = [x - 1 if prec == 4 else x for x in types]
-    x = C.random_input(types, dims, [0, 1, 2], 5)
-    od = C.out_dims(types, dims)
-    xin = x.astype(np.complex64 if np.iscomplexobj(x) else np.float32) if prec == 4 else x
-    got, desc = emu.exec3d(t, dims, [0, 1, 2], od, [0, 1, 2], xin, prec)
-    ref = orc.exec3d(types, dims, [0, 1, 2], [0, 1, 2], x)
+def test_odd_prime_radices_single_buffer(types, dims, prec):
+    """A length with one distinct odd prime factor 7..29 (FP64) or 7..47
+    (FP32) runs on the single-buffer path with that prime as an in-register
+    radix (its own kernel instantiation); FP64 31 runs on the generic path."""
+    err, desc = _run_case(types, dims, prec, 5)
     assert "bluestein" not in desc, desc
-    assert C.rel_l2(got, ref) < (1e-12 if prec == 8 else 1e-5), desc
+    if prec == 4:
+        assert "mid p" in desc, desc
+    assert err < (1e-12 if prec == 8 else 1e-5), desc
 
 
 @pytest.mark.parametrize("prec", [8, 4])
+@pytest.mark.parametrize("types,dims", [
+    ([C.R2C_D, C.CF_D, C.CF_D], [232, 14, 26]),
+    ([C.CF_D] * 3, [31, 17, 19]),
+    ([C.CF_D] * 3, [77, 91, 8]),                    # 7 * 11, 7 * 13: two distinct primes
+])
+def test_odd_prime_radices_generic(types, dims, prec):
+    """Odd prime radices 7..31 as direct DFTs parallel over outputs on the
+    two-buffer generic path (batch plans, and lengths with two distinct odd
+    primes)."""
+    bt = 0 if dims[0] == 77 else 256
+    err, desc = _run_case(types, dims, prec, 5, batch_threads=bt)
+    assert "bluestein" not in desc and "mid p" not in desc, desc
+    assert err < (1e-12 if prec == 8 else 1e-5), desc
+
+
 @pytest.mark.parametrize("types,dims", [
     ([C.R2C_D, C.CF_D, C.CF_D], [344, 6, 4]),       # 8 * 43
     ([C.CF_D] * 3, [37, 41, 47]),
     ([C.r2r_id("DST1", False, 8)] * 3, [36, 7, 5]),  # W = 74 = 2 * 37
 ])
-def test_bluestein_single_buffer(types, dims, prec):
-    """Primes above 31 take Bluestein on the single-buffer path."""
-    t = [x - 1 if prec == 4 else x for x in types]
-    x = C.random_input(types, dims, [0, 1, 2], 6)
-    od = C.out_dims(types, dims)
-    xin = x.astype(np.complex64 if np.iscomplexobj(x) else np.float32) if prec == 4 else x
-    got, desc = emu.exec3d(t, dims, [0, 1, 2], od, [0, 1, 2], xin, prec)
-    ref = orc.exec3d(types, dims, [0, 1, 2], [0, 1, 2], x)
+def test_large_prime_radices_fp32(types, dims):
+    """FP32 primes 37..47 run as in-register radices, not Bluestein."""
+    err, desc = _run_case(types, dims, 4, 6)
+    assert "bluestein" not in desc and "mid p" in desc, desc
+    assert err < 1e-5, desc
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+@pytest.mark.parametrize("types,dims,bt", [
+    ([C.R2C_D, C.CF_D, C.CF_D], [344, 6, 4], 0),    # FP64: 43 above the in-register limit
+    ([C.CF_D] * 3, [53, 59, 61], 0),                # primes above 47
+    ([C.CF_D] * 3, [37, 41, 47], 256),              # batch plans
+    ([C.r2r_id("DST1", False, 8)] * 3, [36, 7, 5], 256),
+])
+def test_bluestein_single_buffer(types, dims, bt, prec):
+    """Other large primes take Bluestein on the single-buffer path."""
+    if prec == 4 and bt == 0 and dims[0] == 344:
+        pytest.skip("FP32 takes 43 as a radix")
+    err, desc = _run_case(types, dims, prec, 6, batch_threads=bt)
     assert "bluestein" in desc, desc
-    assert C.rel_l2(got, ref) < (1e-12 if prec == 8 else 1e-5), desc
+    assert err < (1e-12 if prec == 8 else 1e-5), desc
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+@pytest.mark.parametrize("dims", [[24, 7, 5], [30, 3, 1], [106, 5, 3], [344, 3, 2], [250, 9, 4], [6, 1, 1]])
+@pytest.mark.parametrize("mo2", [[0, 1, 2], [1, 2, 0], [2, 0, 1], [2, 1, 0]])
+def test_r2c_c2r_paired_lines(dims, mo2, prec):
+    """R2C and C2R on the single-buffer path transform two real lines as one
+    complex line: odd line counts (the last line unpaired), tile-order and
+    line-order sides, Bluestein (106 = 2 * 53) and prime radices (344 = 8 * 43,
+    250 = 2 * 5^3); the C2R input is arbitrary complex data, whose DC and
+    Nyquist imaginary parts a real inverse transform ignores."""
+    fwd = [C.R2C_D, C.CF_D, C.CF_D]
+    bwd = [C.C2R_D, C.CB_D, C.CB_D]
+    tf = fwd if prec == 8 else [C.R2C_S, C.CF_S, C.CF_S]
+    tb = bwd if prec == 8 else [C.C2R_S, C.CB_S, C.CB_S]
+    tol = 1e-12 if prec == 8 else 1e-5
+    x = C.random_input(fwd, dims, [0, 1, 2], 21)
+    od = C.out_dims(fwd, dims)
+    xin = x if prec == 8 else x.astype(np.float32)
+    got, desc = emu.exec3d(tf, dims, [0, 1, 2], od, mo2, xin, prec)
+    ref = orc.exec3d(fwd, dims, [0, 1, 2], mo2, x)
+    assert "pair" in desc.split(" | ")[0], desc
+    assert C.rel_l2(got, ref) < tol, desc
+    z = C.random_input(bwd, od, mo2, 22)
+    zin = z if prec == 8 else z.astype(np.complex64)
+    got, desc = emu.exec3d(tb, od, mo2, dims, [0, 1, 2], zin, prec)
+    ref = orc.exec3d(bwd, od, mo2, [0, 1, 2], z)
+    assert C.rel_l2(got, ref) < tol, desc
