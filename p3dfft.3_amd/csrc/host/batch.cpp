@@ -170,7 +170,19 @@ static float time_candidate(int prec, PassDesc &pd, void *din, void *dout, PassA
   return best;
 }
 
-static void tune_pass(int prec, PassDesc &pd, hipStream_t st) {
+// Re-tile pd (tile size B, radix order rad) and give it device tables: a
+// single-buffer pass's twiddles follow the radix order, so retune_pass
+// leaves it without a device copy, and a new one goes to `owned`.
+static bool apply_choice(int prec, PassDesc &pd, int B, const std::vector<int> &rad, std::vector<void *> &owned) {
+  PassDesc t = pd;
+  if (!retune_pass(prec, t, B, rad.data(), (int)rad.size())) return false;
+  if (!tables_ready(t) && !upload_pass_tables(prec, t, owned)) return false;
+  if (!tables_ready(t)) return false;
+  pd = t;
+  return true;
+}
+
+static void tune_pass(int prec, PassDesc &pd, hipStream_t st, std::vector<void *> &owned) {
   PassArgs &a = pd.args;
   if (a.fast || a.cls == CLS_EMPTY) return;
   if (!tables_ready(pd)) return;  // never launch a pass whose tables are not on the device
@@ -178,8 +190,7 @@ static void tune_pass(int prec, PassDesc &pd, hipStream_t st) {
   const std::string key = tune_key(prec, pd);
   auto it = g_tune.find(key);
   if (it != g_tune.end()) {
-    PassDesc t = pd;
-    if (retune_pass(prec, t, it->second.B, it->second.rad.data(), (int)it->second.rad.size())) pd = t;
+    (void)apply_choice(prec, pd, it->second.B, it->second.rad, owned);
     return;
   }
   void *din = nullptr, *dout = nullptr;
@@ -198,10 +209,17 @@ static void tune_pass(int prec, PassDesc &pd, hipStream_t st) {
     float best_ms = time_candidate(prec, pd, din, dout, d_tab, d_work, maxwork, st);
     for (int B : {2, 4, 8, 16, 32}) {
       for (const auto &r : rads) {
+        // a candidate's own tables (its twiddles follow its radix order)
+        // live only while it is timed
+        std::vector<void *> tmp;
         PassDesc t = pd;
-        if (!retune_pass(prec, t, B, r.data(), (int)r.size())) continue;
-        if (t.lds_bytes > 96 * 1024) continue;
+        if (!apply_choice(prec, t, B, r, tmp) || t.lds_bytes > 96 * 1024) {
+          for (void *q : tmp) (void)hipFree(q);
+          continue;
+        }
         const float ms = time_candidate(prec, t, din, dout, d_tab, d_work, maxwork, st);
+        (void)hipStreamSynchronize(st);
+        for (void *q : tmp) (void)hipFree(q);
         if (ms < best_ms) {
           best_ms = ms;
           best = TuneChoice{B, r};
@@ -210,8 +228,7 @@ static void tune_pass(int prec, PassDesc &pd, hipStream_t st) {
     }
     g_tune[key] = best;
     save_cache();
-    PassDesc t = pd;
-    if (retune_pass(prec, t, best.B, best.rad.data(), (int)best.rad.size())) pd = t;
+    (void)apply_choice(prec, pd, best.B, best.rad, owned);
   }
   if (din) (void)hipFree(din);
   if (dout) (void)hipFree(dout);
@@ -272,7 +289,7 @@ BatchPlan *batch_create(int prec, int nitems, const OpSpec ops[3], const int ord
         batch_destroy(b);
         return nullptr;
       }
-      if (tune) tune_pass(prec, pd, st);
+      if (tune) tune_pass(prec, pd, st, b->owned);
       if (!tables_ready(pd)) {
         set_error("batch: tuning lost a constant table");
         batch_destroy(b);

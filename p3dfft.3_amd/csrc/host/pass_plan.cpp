@@ -100,6 +100,27 @@ static cld expi_pi(long long num, long long den) {
   return cld(cosl(a), sinl(a));
 }
 
+// Twiddles of the power-of-two path, pass by pass: for
+// the radix sequence rad, pass i (radix R, stride Ns = the product of the
+// earlier radices) holds w^(k r), w = e^(-2 pi i / (Ns R)), at entry
+// k S + r - 1 (k < Ns, 1 <= r < R), S = tw_stride(R) = R - 1 rounded up to
+// odd.  Consecutive threads of a pass read entries an odd distance apart;
+// in one table of length L they would read entries k r L / (Ns R) apart, a
+// multiple of the LDS bank period for many (Ns, r).  For power-of-two
+// radices the passes' sizes sum to L - 1, and one unused entry pads the
+// table to L.
+static void pass_twiddles(const int *rad, int nrad, std::vector<cld> &tw) {
+  tw.clear();
+  long long Ns = 1;
+  for (int i = 0; i < nrad; i++) {
+    const int R = rad[i], S = p3e::tw_stride(R);
+    for (long long k = 0; k < Ns; k++)
+      for (int r = 1; r <= S; r++) tw.push_back(r < R ? expi_pi(-2LL * k * r, Ns * R) : cld(1, 0));
+    Ns *= R;
+  }
+  tw.push_back(cld(1, 0));
+}
+
 // In-place radix-2 DFT (sign -1) in long double, n a power of two.  Plan-time
 // only: it computes the spectrum of the Bluestein chirp, a constant table.
 static void host_fft_pow2(std::vector<cld> &a) {
@@ -198,26 +219,28 @@ static void setup_mid(int prec, const OpSpec &op, PassDesc &ps, int batch_thread
   const int prime = mid_prime_of(a.radix, a.nrad, prec);
   if (prime < 0 || (prime > 0 && batch_threads > 0)) return;
   const int L = a.L;
-  const long maxth = batch_threads > 0 ? batch_threads : 512;
+  const long maxth = batch_threads > 0 ? batch_threads : P3E_MID_THREADS(prec, prime);
   const size_t esz = 2 * (size_t)prec;
   const size_t per_line = (size_t)a.lpad * esz;
-  size_t twl = batch_threads > 0 ? 0 : (size_t)L * esz;
+  size_t twl = batch_threads > 0 ? 0 : ps.tw.size() * esz;
   if (per_line + twl > LDS_MAX) twl = 0;
   // R2C / C2R: two real lines per LDS line (fft_mid.h, pair_load)
   const bool pair = op.cls == CLS_R2C || op.cls == CLS_C2R;
   auto lines = [&](int b) { return pair ? (b + 1) / 2 : b; };
   int B = 1;
   const int Bmax = std::min(pair ? 64 : 32, next_pow2(a.ne));
-  while (B * 2 <= Bmax && per_line * lines(B * 2) + twl <= LDS_BUDGET && (long)lines(B * 2) * L <= 16 * maxth)
+  const long E = P3E_MID_E;
+  while (B * 2 <= Bmax && per_line * lines(B * 2) + twl <= P3E_MID_LDS && (long)lines(B * 2) * L <= E * maxth)
     B *= 2;
-  if (per_line * lines(B) + twl > LDS_MAX || (long)lines(B) * L > 16 * maxth) return;
-  int th = (int)(((long)lines(B) * L + 15) / 16);
+  if (per_line * lines(B) + twl > LDS_MAX || (long)lines(B) * L > E * maxth) return;
+  int th = (int)(((long)lines(B) * L + E - 1) / E);
   th = std::max(64, (th + 63) & ~63);
   if (th > maxth) return;
   a.mid = 1;
   a.mid_prime = prime;
   a.pair = pair;
   a.tw_in_lds = twl > 0;
+  a.ntw = (int)ps.tw.size();
   a.B = B;
   a.logB = 0;
   while ((1 << a.logB) < B) a.logB++;
@@ -293,8 +316,8 @@ static void setup_fast(int prec, const OpSpec &op, PassDesc &ps, int batch_threa
   ps.threads = M * B / P3E_FAST_E;
   a.nthreads = ps.threads;
   ps.nblocks = (unsigned int)((unsigned long long)a.ntile_e * (unsigned long long)a.nf);
-  ps.tw.resize(M);
-  for (int t = 0; t < M; t++) ps.tw[t] = expi_pi(-2LL * t, M);
+  pass_twiddles(rad, nrad, ps.tw);
+  a.ntw = (int)ps.tw.size();  // M (power-of-two radices): the kernel puts the split factors after it
   ps.wN.clear();
   if (op.cls != CLS_C2C) {
     ps.wN.resize(M + 1);
@@ -387,7 +410,7 @@ bool describe_pass(int prec, int axis, const OpSpec &op, const Layout &lin, cons
     bool ok = false;
     if (batch_threads == 0 && factor(W, rad, mid_max_prime(prec)) &&
         mid_prime_of(rad.data(), (int)rad.size(), prec) > 31 &&
-        W <= 16 * 512 && (size_t)generic_pitch(W) * 2 * (size_t)prec <= LDS_MAX)
+        W <= P3E_MID_E * 512 && (size_t)generic_pitch(W) * 2 * (size_t)prec <= LDS_MAX)
       ok = true;
     if (!ok) ok = factor(W, rad);
     if (!ok) {
@@ -409,6 +432,7 @@ bool describe_pass(int prec, int axis, const OpSpec &op, const Layout &lin, cons
     ps.tw.resize(L);
     for (int t = 0; t < L; t++) ps.tw[t] = expi_pi(-2LL * t, L);
   }
+  a.ntw = (int)ps.tw.size();
   if (a.bluestein) {
     ps.chirp.resize(W);
     std::vector<cld> b(L, cld(0, 0));
@@ -664,8 +688,8 @@ bool retune_pass(int prec, PassDesc &ps, int B, const int *rad, int nrad) {
   const size_t lds = (a.mid ? 1 : 2) * (size_t)a.lpad * 2 * (size_t)prec * nl;
   if (lds > LDS_MAX) return false;
   if (a.mid) {
-    if ((long)nl * a.L > 16L * P3E_STEP_THREADS) return false;
-    const int th = std::max(64, ((int)(((long)nl * a.L + 15) / 16) + 63) & ~63);
+    if ((long)nl * a.L > (long)P3E_MID_E * P3E_STEP_THREADS) return false;
+    const int th = std::max(64, ((int)(((long)nl * a.L + P3E_MID_E - 1) / P3E_MID_E) + 63) & ~63);
     ps.threads = th;
     a.nthreads = th;
   }

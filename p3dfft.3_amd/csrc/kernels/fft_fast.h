@@ -32,7 +32,6 @@ template <typename T, int R, int M, int Ns> struct FPass {
   static constexpr int logR = Log2<R>::v;
   static constexpr int lognbf = Log2<M>::v - logR;
   static constexpr int nbf = M / R;
-  static constexpr int tstride = M / (Ns * R);  // twiddle index stride
   cx<T> *buf;
   int lp, nb, s;
   const cx<T> *tw;
@@ -62,7 +61,7 @@ template <typename T, int R, int M, int Ns> struct FPass {
         cx<T> *w = v + m * R;
         if (Ns > 1) {
 #pragma unroll
-          for (int r = 1; r < R; r++) w[r] = mul(w[r], twid(tw, k * r * tstride, s));
+          for (int r = 1; r < R; r++) w[r] = mul(w[r], twid(tw, k * tw_stride(R) + r - 1, s));
         }
         bfly<T, R>(w, s);
         cx<T> *dl = buf + (b >> lognbf) * lp;
@@ -75,12 +74,14 @@ template <typename T, int R, int M, int Ns> struct FPass {
 };
 
 // Run the radix sequence R, Rest... starting at stride Ns; `run` executes
-// one pass (read half, barrier, write half, barrier).
+// one pass (read half, barrier, write half, barrier).  tw: the twiddles of
+// this pass onward, pass by pass (pass_twiddles on the host): Ns
+// tw_stride(R) entries for a radix-R pass at stride Ns.
 template <typename T, int M, int Ns, typename Run, int R, int... Rest>
 P3E_HD P3E_INL void fft_seq(Run &run, cx<T> *buf, int lp, int nb, int s, const cx<T> *tw) {
   FPass<T, R, M, Ns> ps{buf, lp, nb, s, tw};
   run(ps);
-  if constexpr (sizeof...(Rest) > 0) fft_seq<T, M, Ns * R, Run, Rest...>(run, buf, lp, nb, s, tw);
+  if constexpr (sizeof...(Rest) > 0) fft_seq<T, M, Ns * R, Run, Rest...>(run, buf, lp, nb, s, tw + Ns * tw_stride(R));
 }
 
 // Radix sequence of each supported length (all radices divide P3E_FAST_E).

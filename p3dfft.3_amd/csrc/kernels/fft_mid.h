@@ -9,7 +9,7 @@
 // Stockham positions, barrier.  Half the LDS of the two-buffer generic path
 // per line, so twice the lines per tile (longer contiguous runs on a
 // strided axis).  The planner sizes the workgroup so that every thread owns
-// at most 16 elements of the tile (nb * L <= 16 * nth).  The prime P is a
+// at most P3E_MID_E elements of the tile (nb * L <= P3E_MID_E * nth).  The prime P is a
 // template parameter (PassArgs::mid_prime selects the instantiation), so
 // the registers of its butterfly are only allocated in the kernels that
 // need it.
@@ -21,7 +21,7 @@ namespace p3e {
 
 // Radix-R pass at input stride Ns over nb padded lines of length L.
 template <typename T, int R> struct MPass {
-  static constexpr int NB = (16 + R - 1) / R;  // butterflies per thread, at most
+  static constexpr int NB = (P3E_MID_E + R - 1) / R;  // butterflies per thread, at most
   static constexpr int SLOTS = NB * R;
   cx<T> *buf;
   int L, lpad, nb, Ns, s;
@@ -73,7 +73,10 @@ template <typename T, int R> struct MPass {
 };
 
 // FFT of length L on nb lines in place; `run` executes one pass (read half,
-// barrier, write half, barrier).
+// barrier, write half, barrier).  tw: one table of length L, w^(k r) of a
+// radix-R pass at stride Ns at entry k r L / (Ns R).  (The pass-by-pass
+// layout of the power-of-two path measured slower here on MI355X for
+// lengths with radices 3 and 5, and faster only for power-of-two L.)
 template <typename T, int P, typename Run>
 P3E_HD P3E_INL void mid_fft(Run &run, cx<T> *buf, const PassArgs &p, int nb, int s, const cx<T> *tw) {
   int Ns = 1;

@@ -44,9 +44,11 @@ __global__ void __launch_bounds__(256) pass_kernel(PassArgs p) {
   const cx<T> *tw = reinterpret_cast<const cx<T> *>(p.tw);
   if (p.tw_in_lds) {
     cx<T> *ltw = bufB + (size_t)p.B * p.lpad;
-    for (int i = threadIdx.x; i < p.L; i += blockDim.x) ltw[i] = tw[i];
+    for (int i = threadIdx.x; i < p.ntw; i += blockDim.x) ltw[i] = tw[i];
     tw = ltw;
   }
+  // one call site: a second inlined copy of the tile code makes the
+  // compiler keep PassArgs in scratch memory
   run_tile<T>(p, xcd_tile(blockIdx.x, gridDim.x), bufA, bufB, threadIdx.x, blockDim.x, DevBarrier(), tw);
 }
 
@@ -89,7 +91,7 @@ __global__ void __launch_bounds__(P3E_FAST_MAX_THREADS) pass_kernel_pow2(PassArg
   const cx<T> *gtw = reinterpret_cast<const cx<T> *>(p0.tw);
   const cx<T> *gwn = reinterpret_cast<const cx<T> *>(p0.wN);
   const int tid = threadIdx.x, nth = blockDim.x;
-  for (int i = tid; i < M; i += nth) ltw[i] = gtw[i];
+  for (int i = tid; i < p0.ntw; i += nth) ltw[i] = gtw[i];
   if (p0.cls != CLS_C2C) {
     for (int i = tid; i <= M; i += nth) lwn[i] = gwn[i];
     // the C2R load phase merges (X_k, X_{M-k}) with the split factors, so
@@ -107,18 +109,21 @@ __global__ void __launch_bounds__(P3E_FAST_MAX_THREADS) pass_kernel_pow2(PassArg
 // Single-buffer mixed-radix path: one padded LDS buffer, then the twiddles
 // when they fit.  P: the odd prime radix of the pass (0: none).
 template <typename T, int P>
-__global__ void __launch_bounds__(512) pass_kernel_mid(PassArgs p) {
+__global__ void __launch_bounds__(P3E_MID_THREADS(sizeof(T), P)) pass_kernel_mid(PassArgs p) {
   extern __shared__ unsigned char smem_raw[];
   cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
   const cx<T> *tw = reinterpret_cast<const cx<T> *>(p.tw);
   const int tid = threadIdx.x, nth = blockDim.x;
   if (p.tw_in_lds) {
     cx<T> *ltw = buf + (size_t)tile_lines(p, p.B) * p.lpad;
-    for (int i = tid; i < p.L; i += nth) ltw[i] = tw[i];
+    for (int i = tid; i < p.ntw; i += nth) ltw[i] = tw[i];
     tw = ltw;
   }
   DevPassRun<T> run{tid, nth};
   DevEach each{tid};
+  // one call site (the twiddle reads then compile to flat loads, as tw may
+  // point to LDS or to global memory): two inlined copies, each with one
+  // address space, measured slower on MI355X
   mid_tile<T, P>(p, xcd_tile(blockIdx.x, gridDim.x), buf, nth, run, each, tw);
 }
 
@@ -266,7 +271,7 @@ extern "C" int p3e_launch_pass(const p3e::PassArgs *a, int prec, size_t lds_byte
                      : p3e::launch_pow2_any<float>(a, lds_bytes, threads, nblocks, st);
   }
   if (a->mid) {
-    if (threads < 1 || threads > 512) return (int)hipErrorInvalidValue;
+    if (threads < 1 || threads > P3E_MID_THREADS(prec, a->mid_prime)) return (int)hipErrorInvalidValue;
     return prec == 8 ? p3e::launch_mid_any<double>(a, lds_bytes, threads, nblocks, st)
                      : p3e::launch_mid_any<float>(a, lds_bytes, threads, nblocks, st);
   }

@@ -76,6 +76,7 @@ struct PassArgs {
   int nthreads;      // threads that work on a tile (a batch launch may have more)
   FastDiv div_pairs; // C2R: division by M/2 + 1, the pairs (X_k, X_{M-k}) per line
   int tw_in_lds;     // generic path: the standalone kernel copies tw to LDS
+  int ntw;           // entries of the twiddle table tw
   int mid;           // single-buffer mixed-radix path (fft_mid.h)
   int mid_prime;     // its odd prime radix 7..47 (0: none); selects the kernel instantiation
   int pair;          // R2C/C2R on the single-buffer path: lines 2t, 2t+1 share LDS line t
@@ -102,6 +103,21 @@ struct WorkItem {
 
 // elements held in registers per thread by the fast path
 #define P3E_FAST_E 16
+// Single-buffer mixed-radix path: elements of the tile per thread, at most
+#ifndef P3E_MID_E
+#define P3E_MID_E 16
+#endif
+// and its LDS budget per workgroup in bytes (a tile may exceed it up to the
+// 160 KiB of a CU when one line alone does)
+#ifndef P3E_MID_LDS
+#define P3E_MID_LDS (80 * 1024)
+#endif
+// and its largest workgroup for FP32 passes without an odd prime radix
+// (the others, whose butterflies need more registers, use 512)
+#ifndef P3E_MID_MAXTH
+#define P3E_MID_MAXTH 512
+#endif
+#define P3E_MID_THREADS(esz, prime) (((esz) == 4 && (prime) == 0) ? P3E_MID_MAXTH : 512)
 // threads per workgroup of the fast path, at most
 #define P3E_FAST_MAX_THREADS 512
 
@@ -125,5 +141,10 @@ static inline int fast_radices(int M, int *rad) {
 // padded LDS index within a line (both paths): one spare slot every 16
 // elements, so that strided accesses of a radix pass spread over the banks
 #define P3E_PAD(i) ((i) + ((i) >> 4))
+
+// Entries per k of a radix-R pass's twiddle table in the power-of-two path:
+// R - 1 rounded up to odd, so that consecutive k fall on different LDS
+// banks.
+constexpr int tw_stride(int R) { return (R - 1) | 1; }
 
 }  // namespace p3e
