@@ -161,14 +161,14 @@ def poisson_ref64(torch, rhs, d):
     return torch.fft.irfftn(F * fac, s=(nz, ny, nx), dim=(0, 1, 2))
 
 
-def serving_bench(P, torch, rates=(50, 200, 800), seconds=(4.0, 2.0, 1.0), nshapes=32, closed=256):
+def serving_bench(P, torch, rates=(50, 200, 800), seconds=(4.0, 2.0, 1.0), nshapes=32, closed=256, max_inflight=64):
     """Periodic Poisson solves (FP32) through the Poisson service: open-loop
     Poisson arrivals at each rate (latency p50/p99, solves/s), and the
     closed-loop capacity against the per-request path."""
     import numpy as np
     shapes = poisson_shapes(nshapes, 5)
     rhs = [poisson_rhs(torch, d, 100 + i) for i, d in enumerate(shapes)]
-    h = P.poisson_server_create(4, 48 << 30, 64)
+    h = P.poisson_server_create(4, 48 << 30, max_inflight)
     for d in shapes:
         P.poisson_server_prepare(h, d)
     torch.cuda.synchronize()

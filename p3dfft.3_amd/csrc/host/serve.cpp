@@ -123,6 +123,7 @@ static void retire(PoissonServer *s, PoissonServer::Step &b) {
     r->t_done = t;
     pool_free(s, r->off, 2 * r->plan->work_bytes);
     s->holding--;
+    if (r->plan->dedicated) s->holding_large--;
     s->completed.push_back(r);
   }
   b.finishing.clear();
@@ -206,9 +207,11 @@ static void service_loop(PoissonServer *s) {
       PoissonServer::Step &b = s->steps[(s->cur + k) % SERVE_DEPTH];
       if (b.busy && hipEventQuery(b.done) == hipSuccess) retire(s, b);
     }
-    // admission: arrival order, a slot and working memory
+    // admission: arrival order, a slot, working memory, and for a large
+    // grid one of LARGE_INFLIGHT places
     while (!s->pending.empty() && s->holding < s->max_inflight) {
       PoissonRequest *r = s->pending.front();
+      if (r->plan->dedicated && s->holding_large >= LARGE_INFLIGHT) break;
       size_t off;
       if (!pool_alloc(s, 2 * r->plan->work_bytes, &off)) break;
       r->off = off;
@@ -217,6 +220,7 @@ static void service_loop(PoissonServer *s) {
       s->pending.pop_front();
       s->active.push_back(r);
       s->holding++;
+      if (r->plan->dedicated) s->holding_large++;
     }
     if (s->active.empty()) {
       PoissonServer::Step *oldest = nullptr;
@@ -280,6 +284,7 @@ PoissonServer *poisson_server_create(int prec, size_t pool_bytes, int max_inflig
   s->st = nullptr;
   s->cur = 0;
   s->holding = 0;
+  s->holding_large = 0;
   s->next_id = 0;
   s->nsteps = s->ntiles = 0;
   s->stop = s->failed = false;

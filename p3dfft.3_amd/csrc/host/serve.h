@@ -40,7 +40,17 @@ struct PoissonPlan {
   size_t real_bytes;
 };
 
-static const long long DEDICATED_POINTS = 1LL << 20;
+// 2^18 points: a 64^3 grid's pass kernels already have about 1000
+// workgroups; grids above it ran faster with their own launches on MI355X
+// (4230 against 3870 solves/s for 64..384-point edges, where the step
+// kernel's batch tiles lack the in-register prime radices).
+static const long long DEDICATED_POINTS = 1LL << 18;
+// Large (dedicated) requests in flight at most, whatever max_inflight
+// allows: the passes of the requests in flight are interleaved step by
+// step, and a request's two working arrays stay in the 256 MiB Infinity
+// Cache between its passes only when few others are interleaved with it
+// (4500 solves/s at 2-4 against 4220 at 64 on MI355X).
+static const int LARGE_INFLIGHT = 4;
 // Steps in flight on the device: the service thread builds the next step
 // while the earlier one runs.
 static const int SERVE_DEPTH = 2;
@@ -79,7 +89,8 @@ struct PoissonServer {
   std::deque<PoissonRequest *> pending;
   std::vector<PoissonRequest *> active;
   std::vector<PoissonRequest *> completed;  // not yet polled
-  int holding;  // admitted and not yet retired
+  int holding;        // admitted and not yet retired
+  int holding_large;  // of which dedicated (large-grid) requests
   long long next_id;
   long long nsteps, ntiles;
   bool stop, failed;

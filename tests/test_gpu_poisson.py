@@ -21,7 +21,10 @@ pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
 TOL = {8: 1e-12, 4: 1e-5}
-SIZES = [[64, 64, 64], [72, 48, 96], [40, 56, 48], [74, 64, 64], [128, 64, 96], [88, 88, 48]]
+# grids of 2^18 points or more run dedicated pass launches in the service,
+# smaller ones ([40, 56, 48], [48, 40, 36], [56, 32, 40]) share step-kernel launches
+SIZES = [[64, 64, 64], [72, 48, 96], [40, 56, 48], [74, 64, 64], [128, 64, 96], [88, 88, 48], [48, 40, 36],
+         [56, 32, 40]]
 
 
 def log(*a):
@@ -159,13 +162,13 @@ def test_service_rejects_oversized_request():
 
 
 def test_service_large_grid_dedicated_launches():
-    """A grid of at least 2^20 points runs its passes as their own launches,
+    """A grid of at least 2^18 points runs its passes as their own launches,
     mixed with small requests in the same steps: parity with the oracle, and
     the same bits alone and in the mix."""
     big = [256, 256, 128]
     log("case: service large grid", big)
     alone, _, _ = run_server([(big, 30)], 4)
-    reqs = [(SIZES[0], 31), (big, 30), (SIZES[1], 32), (SIZES[2], 33)]
+    reqs = [(SIZES[6], 31), (big, 30), (SIZES[1], 32), (SIZES[2], 33), (SIZES[7], 34)]
     mix, _, st = run_server(reqs, 4)
     log("  steps", st)
     assert np.array_equal(alone[0], mix[1])
