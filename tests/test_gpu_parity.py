@@ -355,3 +355,75 @@ def test_config2_512cubed_roundtrip_and_derivative(prec):
         assert err < 1e-12
     else:
         assert rel < 5e-5
+
+
+PRIMES = [7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47]
+
+
+def _path_of_first_pass(pl):
+    return P.describe(pl.h).split(" | ")[0]
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+@pytest.mark.parametrize("p", PRIMES)
+def test_prime_radix_kernels(p, prec):
+    """Every in-register odd prime radix of the single-buffer path (one
+    kernel per prime: 7..47 in FP32, 7..29 in FP64; FP64 31 on the generic
+    path, FP64 37..47 by Bluestein) on a C2C of length 4p, against the
+    oracle."""
+    dims = [4 * p, 6, 5]
+    types = [C.CF_D] * 3
+    x = C.random_input(types, dims, [0, 1, 2], 100 + p)
+    t = types if prec == 8 else [C.CF_S] * 3
+    xin = x if prec == 8 else x.astype(np.complex64)
+    got, pl = gpu_3d(t, dims, [0, 1, 2], [1, 2, 0], xin)
+    ref = orc.exec3d(types, dims, [0, 1, 2], [1, 2, 0], x)
+    first = _path_of_first_pass(pl)
+    log("case: prime", p, "prec", prec, first)
+    if prec == 4 or p <= 29:
+        assert f"mid p{p}" in first, first
+    elif p == 31:
+        assert "mid" not in first and "bluestein" not in first, first
+    else:
+        assert "bluestein" in first, first
+    assert C.rel_l2(got, ref) < TOL[prec]
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+@pytest.mark.parametrize("n,mo2", [(2 * 17, [0, 1, 2]), (2 * 29, [1, 2, 0]), (4 * 43, [2, 1, 0]), (2 * 53, [1, 2, 0]),
+                                   (250, [2, 0, 1]), (24, [0, 2, 1])])
+def test_paired_real_lines(n, mo2, prec):
+    """R2C and C2R on the single-buffer path transform two real lines per
+    complex FFT: an odd number of lines (7, the last unpaired), line- and
+    tile-order sides, prime radices and Bluestein (2 * 53); the C2R input is
+    arbitrary complex data (its DC and Nyquist imaginary parts are ignored,
+    as by a real inverse transform)."""
+    dims = [n, 7, 3]
+    fwd = [C.R2C_D, C.CF_D, C.CF_D]
+    bwd = [C.C2R_D, C.CB_D, C.CB_D]
+    tf = fwd if prec == 8 else [C.R2C_S, C.CF_S, C.CF_S]
+    tb = bwd if prec == 8 else [C.C2R_S, C.CB_S, C.CB_S]
+    x = C.random_input(fwd, dims, [0, 1, 2], 31)
+    got, pl = gpu_3d(tf, dims, [0, 1, 2], mo2, x if prec == 8 else x.astype(np.float32))
+    first = _path_of_first_pass(pl)
+    log("case: paired", n, mo2, prec, first)
+    assert "pair" in first, first
+    assert C.rel_l2(got, orc.exec3d(fwd, dims, [0, 1, 2], mo2, x)) < TOL[prec]
+    od = C.out_dims(fwd, dims)
+    z = C.random_input(bwd, od, mo2, 32)
+    got, pl = gpu_3d(tb, od, mo2, [0, 1, 2], z if prec == 8 else z.astype(np.complex64), c2r_len=n)
+    assert C.rel_l2(got, orc.exec3d(bwd, od, mo2, [0, 1, 2], z)) < TOL[prec]
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+def test_bluestein_above_47(prec):
+    """Primes above 47 (53, 59, 61) take Bluestein on the single-buffer path."""
+    dims = [53, 59, 61]
+    types = [C.CF_D] * 3
+    x = C.random_input(types, dims, [0, 1, 2], 7)
+    t = types if prec == 8 else [C.CF_S] * 3
+    got, pl = gpu_3d(t, dims, [0, 1, 2], [0, 1, 2], x if prec == 8 else x.astype(np.complex64))
+    desc = P.describe(pl.h)
+    log("case: bluestein 53 59 61", prec, desc)
+    assert desc.count("bluestein") == 3, desc
+    assert C.rel_l2(got, orc.exec3d(types, dims, [0, 1, 2], [0, 1, 2], x)) < TOL[prec]
