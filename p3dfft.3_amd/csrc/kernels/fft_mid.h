@@ -102,6 +102,64 @@ P3E_HD P3E_INL void mid_fft(Run &run, cx<T> *buf, const PassArgs &p, int nb, int
   }
 }
 
+// ---------------------------------------------------------------- load phase
+// Load phase of the single-buffer path in groups: every thread issues the
+// global loads of U entries into registers, then scatters the U values into
+// LDS, so that U loads are in flight together (a loop of load-then-scatter
+// keeps about one in flight; all of a thread's entries at once, as in the
+// power-of-two path, unrolls the scatter rules too far and spills).  Entry
+// u of a group at i0 is tile element i0 + u nth; out-of-tile entries load
+// the tile's first element (always valid) and are not scattered.  Values
+// are kept as scalars: arrays of cx<double> would not be split into
+// registers.
+
+// (line j, element n) of tile element i in load order, over lines of len
+// elements (line order) or rows of 2^logw lines (tile order).
+P3E_HD P3E_INL void mid_load_index(bool line_order, int i, int len, const FastDiv &dlen, int logw, int *j, int *n) {
+  if (line_order) {
+    *j = (int)fdiv((uint32_t)i, dlen);
+    *n = i - *j * len;
+  } else {
+    *j = i & ((1 << logw) - 1);
+    *n = i >> logw;
+  }
+}
+
+template <typename T>
+P3E_HD P3E_INL void mid_load(const PassArgs &p, uint32_t f, int e0, int nb, cx<T> *buf, int tid, int nth) {
+  constexpr int U = 4;  // loads in flight per thread
+  const cx<T> *chirp = reinterpret_cast<const cx<T> *>(p.chirp);
+  const cx<T> *pre = reinterpret_cast<const cx<T> *>(p.pre);
+  const long long base = (long long)e0 * p.is_e + (long long)f * p.is_f;
+  const bool lo = p.load_order == ORD_LINE;
+  const int tot = (lo ? nb : p.B) * p.nin;
+  for (int i0 = tid; i0 < tot; i0 += U * nth) {
+    T v[2 * U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = i0 + u * nth;
+      int j, n;
+      mid_load_index(lo, i, p.nin, p.div_nin, p.logB, &j, &n);
+      const long long off = (i < tot && j < nb) ? base + (long long)n * p.is_d + (long long)j * p.is_e : base;
+      if (p.in_cplx) {
+        const cx<T> x = reinterpret_cast<const cx<T> *>(p.in)[off];
+        v[2 * u] = x.x;
+        v[2 * u + 1] = x.y;
+      } else {
+        v[2 * u] = reinterpret_cast<const T *>(p.in)[off];
+        v[2 * u + 1] = (T)0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = i0 + u * nth;
+      int j, n;
+      mid_load_index(lo, i, p.nin, p.div_nin, p.logB, &j, &n);
+      if (i < tot && j < nb) scatter(buf + j * p.lpad, n, mk(v[2 * u], v[2 * u + 1]), p, chirp, pre);
+    }
+  }
+}
+
 // ---------------------------------------------------------------- paired lines
 // R2C and C2R with p.pair: one complex FFT of length W = N serves two real
 // lines.  LDS line t holds z = x_{2t} + i x_{2t+1} (line 2t+1 absent -> 0).
@@ -116,36 +174,56 @@ P3E_HD P3E_INL int tile_lines(const PassArgs &p, int nb) { return p.pair ? (nb +
 
 template <typename T>
 P3E_HD P3E_INL void pair_load(const PassArgs &p, uint32_t f, int e0, int nb, cx<T> *buf, int tid, int nth) {
+  constexpr int U = 4;  // entries (line pairs at one n) loaded together, as in mid_load
   const cx<T> *chirp = reinterpret_cast<const cx<T> *>(p.chirp);
   const long long base = (long long)e0 * p.is_e + (long long)f * p.is_f;
   const int nl = (nb + 1) / 2, N = p.N;
   const int lbl = p.logB > 0 ? p.logB - 1 : 0;  // B / 2 line pairs per row (1 if B = 1)
-  const int tot = (p.load_order == ORD_LINE ? nl : 1 << lbl) * p.nin;
-  for (int i = tid; i < tot; i += nth) {
-    int n, t;
-    if (p.load_order == ORD_LINE) {
-      t = (int)fdiv((uint32_t)i, p.div_nin);
-      n = i - t * p.nin;
-    } else {
-      t = i & ((1 << lbl) - 1);
-      n = i >> lbl;
-      if (t >= nl) continue;
-    }
-    const bool two = 2 * t + 1 < nb;
-    const long long off = base + (long long)n * p.is_d + (long long)(2 * t) * p.is_e;
-    cx<T> *line = buf + t * p.lpad;
-    if (p.cls == CLS_R2C) {
-      const T *in = reinterpret_cast<const T *>(p.in);
-      put(line, n, mk(in[off], two ? in[off + p.is_e] : (T)0), p, chirp);
-    } else {
-      const cx<T> *in = reinterpret_cast<const cx<T> *>(p.in);
-      cx<T> a = in[off], b = two ? in[off + p.is_e] : mk((T)0, (T)0);
-      if (n == 0 || 2 * n == N) {
-        a.y = (T)0;
-        b.y = (T)0;
+  const bool lo = p.load_order == ORD_LINE;
+  const int tot = (lo ? nl : 1 << lbl) * p.nin;
+  for (int i0 = tid; i0 < tot; i0 += U * nth) {
+    // entry u: element n of the lines 2t (a) and 2t+1 (b, 0 if absent)
+    T v[4 * U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = i0 + u * nth;
+      int t, n;
+      mid_load_index(lo, i, p.nin, p.div_nin, lbl, &t, &n);
+      const bool ok = i < tot && t < nl, two = ok && 2 * t + 1 < nb;
+      const long long off = ok ? base + (long long)n * p.is_d + (long long)(2 * t) * p.is_e : base;
+      const long long off2 = two ? off + p.is_e : base;
+      if (p.cls == CLS_R2C) {
+        const T *in = reinterpret_cast<const T *>(p.in);
+        const T a = in[off], b = in[off2];
+        v[4 * u] = a;
+        v[4 * u + 1] = two ? b : (T)0;
+      } else {
+        const cx<T> *in = reinterpret_cast<const cx<T> *>(p.in);
+        const cx<T> a = in[off], b = in[off2];
+        v[4 * u] = a.x;
+        v[4 * u + 1] = a.y;
+        v[4 * u + 2] = two ? b.x : (T)0;
+        v[4 * u + 3] = two ? b.y : (T)0;
       }
-      put(line, n, mk(a.x - b.y, a.y + b.x), p, chirp);
-      if (n >= 1 && n < (N + 1) / 2) put(line, N - n, mk(a.x + b.y, b.x - a.y), p, chirp);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = i0 + u * nth;
+      int t, n;
+      mid_load_index(lo, i, p.nin, p.div_nin, lbl, &t, &n);
+      if (!(i < tot && t < nl)) continue;
+      cx<T> *line = buf + t * p.lpad;
+      if (p.cls == CLS_R2C) {
+        put(line, n, mk(v[4 * u], v[4 * u + 1]), p, chirp);
+      } else {
+        cx<T> a = mk(v[4 * u], v[4 * u + 1]), b = mk(v[4 * u + 2], v[4 * u + 3]);
+        if (n == 0 || 2 * n == N) {
+          a.y = (T)0;
+          b.y = (T)0;
+        }
+        put(line, n, mk(a.x - b.y, a.y + b.x), p, chirp);
+        if (n >= 1 && n < (N + 1) / 2) put(line, N - n, mk(a.x + b.y, b.x - a.y), p, chirp);
+      }
     }
   }
 }
@@ -212,7 +290,7 @@ P3E_HD P3E_INL void mid_tile(const PassArgs &p, uint32_t tile, cx<T> *buf, int n
   each([&](int tid) {
     if (tid < nth) {
       if (p.pair) pair_load<T>(p, f, e0, nb, buf, tid, nth);
-      else generic_load<T>(p, f, e0, nb, buf, tid, nth);
+      else mid_load<T>(p, f, e0, nb, buf, tid, nth);
     }
   });
   if (p.cls != CLS_EMPTY) {
