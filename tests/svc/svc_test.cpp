@@ -123,6 +123,7 @@ static int scenario(int max_inflight, size_t pool, int nreq, unsigned seed) {
   });
   // submit in bursts, polling between, as a client would
   std::map<long long, int> done;
+  std::map<long long, std::pair<double, double>> span;  // id -> (admitted, done)
   auto t0 = std::chrono::steady_clock::now();
   int next = 0;
   std::vector<long long> ids(nreq, -1);
@@ -144,6 +145,7 @@ static int scenario(int max_inflight, size_t pool, int nreq, unsigned seed) {
         return 1;
       }
       done[got[i]] = 1;
+      span[got[i]] = {ta[i], td[i]};
     }
     if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(60)) {
       printf("HANG: %zu of %d requests done (max_inflight %d)\n", done.size(), nreq, max_inflight);
@@ -156,6 +158,23 @@ static int scenario(int max_inflight, size_t pool, int nreq, unsigned seed) {
   dog.join();
   poisson_server_destroy(s);
   mock_drain();
+  // large grids (dedicated launches): at most LARGE_INFLIGHT held at once
+  {
+    std::vector<std::pair<double, int>> ev;  // (time, +1 admit / -1 done), done first at equal times
+    for (int i = 0; i < nreq; i++) {
+      const long long pts = (long long)reqs[i].dims[0] * reqs[i].dims[1] * reqs[i].dims[2];
+      if (pts < DEDICATED_POINTS) continue;
+      ev.push_back({span[ids[i]].first, +1});
+      ev.push_back({span[ids[i]].second, -1});
+    }
+    std::sort(ev.begin(), ev.end());
+    int cur = 0, peak = 0;
+    for (auto &e : ev) peak = std::max(peak, cur += e.second);
+    if (peak > LARGE_INFLIGHT) {
+      printf("%d large requests held at once (limit %d)\n", peak, LARGE_INFLIGHT);
+      return 6;
+    }
+  }
   // chains: every request's passes, in device order, rhs -> ... -> sol
   std::vector<Exec> ex;
   {

@@ -2,8 +2,9 @@
 CPU against a mock HIP runtime whose device thread executes the stream's
 commands asynchronously (tests/svc): for several in-flight limits, pool
 sizes and device speeds, every request completes, each request's passes
-form the chain rhs -> ... -> solution in device order, and no other pass
-writes a buffer between its writer and its reader."""
+form the chain rhs -> ... -> solution in device order, no other pass
+writes a buffer between its writer and its reader, and no more than four
+large-grid requests (dedicated launches) hold working memory at once."""
 import os
 import subprocess
 
