@@ -126,7 +126,7 @@ def test_r2r_inverse_pairs():
 
 def test_type_registry_order_and_dct4_quirk():
     assert orc.lib().or_num_types() == 44
-    # build/init.C:121-731 registration order (Appendix B of the survey)
+    # build/init.C:121-731 registration order
     assert orc.type_info(5).cls == 1 and orc.type_info(5).prec == 8  # R2CFFT_D
     assert orc.type_info(6).cls == 2 and orc.type_info(6).prec == 4  # C2RFFT_S
     assert orc.type_info(9).isign == -1 and orc.type_info(11).isign == 1
