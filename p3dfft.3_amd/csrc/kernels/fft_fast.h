@@ -169,6 +169,11 @@ P3E_HD P3E_INL void c2r_pair_put(const PassArgs &p, cx<T> *line, int k, cx<T> xk
   if (2 * k != M) line[P3E_PAD(M - k)] = add(add(xm, conj(xk)), rot(mul(conj(wN[M - k]), sub(xm, conj(xk))), 1));
 }
 
+// Two reals in one load (8 or 16 bytes, aligned to its size).
+template <typename T> struct alignas(2 * sizeof(T)) RealPair {
+  T a, b;
+};
+
 template <typename T, int M>
 P3E_HD P3E_INL void fast_fetch(const PassArgs &p, uint32_t f, int e0, int nb, TileRegs<T> &R, int tid, int nth) {
   constexpr int E = P3E_FAST_E;
@@ -176,7 +181,23 @@ P3E_HD P3E_INL void fast_fetch(const PassArgs &p, uint32_t f, int e0, int nb, Ti
   const int tot = (p.load_order == ORD_LINE ? nb : p.B) * p.nin;
   // out-of-tile entries load the tile's first element (always valid) and
   // are not written: unconditional loads keep the values in registers
-  if (p.cls == CLS_R2C) {
+  if (p.vec2) {
+    // R2C, tile order, e stride-1 and whole tiles (the planner's condition):
+    // entry q is element n of lines j = 2 jp and j + 1, adjacent in memory;
+    // the launcher checked that base and strides keep the pair aligned
+    const T *in = reinterpret_cast<const T *>(p.in);
+    const int lb = p.logB - 1;
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+      const int i = tid + q * nth;
+      const int jp = i & ((1 << lb) - 1), n = i >> lb;
+      const bool ok = i < (p.B >> 1) * p.nin;
+      const RealPair<T> x =
+          *reinterpret_cast<const RealPair<T> *>(in + (ok ? base + (long long)n * p.is_d + 2 * jp : base));
+      R.v[2 * q] = x.a;
+      R.v[2 * q + 1] = x.b;
+    }
+  } else if (p.cls == CLS_R2C) {
     const T *in = reinterpret_cast<const T *>(p.in);
 #pragma unroll
     for (int q = 0; q < 2 * E; q++) {
@@ -224,7 +245,20 @@ P3E_HD P3E_INL void fast_put(const PassArgs &p, uint32_t f, int e0, int nb, cons
                              int tid, int nth) {
   constexpr int E = P3E_FAST_E;
   const int tot = (p.load_order == ORD_LINE ? nb : p.B) * p.nin;
-  if (p.cls == CLS_R2C) {
+  if (p.vec2) {
+    T *bufr = reinterpret_cast<T *>(buf);
+    const int lb = p.logB - 1;
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+      const int i = tid + q * nth;
+      const int j = 2 * (i & ((1 << lb) - 1)), n = i >> lb;
+      if (i < (p.B >> 1) * p.nin) {
+        const int o = 2 * P3E_PAD(n >> 1) + (n & 1);
+        bufr[2 * j * p.lp + o] = R.v[2 * q];
+        bufr[2 * (j + 1) * p.lp + o] = R.v[2 * q + 1];
+      }
+    }
+  } else if (p.cls == CLS_R2C) {
     T *bufr = reinterpret_cast<T *>(buf);
 #pragma unroll
     for (int q = 0; q < 2 * E; q++) {

@@ -77,6 +77,7 @@ struct PassArgs {
   FastDiv div_pairs; // C2R: division by M/2 + 1, the pairs (X_k, X_{M-k}) per line
   int tw_in_lds;     // generic path: the standalone kernel copies tw to LDS
   int ntw;           // entries of the twiddle table tw
+  int vec2;          // power-of-two R2C, tile-order load: one load reads lines j, j+1 (fft_fast.h)
   int mid;           // single-buffer mixed-radix path (fft_mid.h)
   int mid_prime;     // its odd prime radix 7..47 (0: none); selects the kernel instantiation
   int pair;          // R2C/C2R on the single-buffer path: lines 2t, 2t+1 share LDS line t

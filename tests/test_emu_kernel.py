@@ -292,3 +292,20 @@ def test_r2c_c2r_paired_lines(dims, mo2, prec):
     got, desc = emu.exec3d(tb, od, mo2, dims, [0, 1, 2], zin, prec)
     ref = orc.exec3d(bwd, od, mo2, [0, 1, 2], z)
     assert C.rel_l2(got, ref) < tol, desc
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+@pytest.mark.parametrize("dims,mo1,mo2", [([64, 32, 64], [2, 1, 0], [1, 2, 0]), ([128, 16, 32], [2, 1, 0], [2, 1, 0]),
+                                          ([32, 8, 48], [1, 2, 0], [1, 2, 0])])
+def test_r2c_tile_load_real_pairs(dims, mo1, mo2, prec):
+    """Power-of-two R2C with a tile-order load along a stride-1 axis reads two
+    adjacent lines' reals per load when every tile is whole and the strides
+    are even (the first two cases); the third has a partial last tile and
+    loads one real at a time."""
+    fwd = [C.R2C_D, C.CF_D, C.CF_D]
+    t = fwd if prec == 8 else [C.R2C_S, C.CF_S, C.CF_S]
+    x = C.random_input(fwd, dims, mo1, 3)
+    od = C.out_dims(fwd, dims)
+    got, desc = emu.exec3d(t, dims, mo1, od, mo2, x if prec == 8 else x.astype(np.float32), prec)
+    assert "pow2" in desc.split(" | ")[0], desc
+    assert C.rel_l2(got, orc.exec3d(fwd, dims, mo1, mo2, x)) < (1e-12 if prec == 8 else 1e-5), desc
