@@ -313,7 +313,34 @@ P3E_HD void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, const cx<T
   const long long base = (long long)e0 * p.os_e + (long long)f * p.os_f;
   const int tot = (p.store_order == ORD_LINE ? nb : p.B) * p.nout;
   constexpr int E = P3E_FAST_E;
-  if (p.cls == CLS_C2R) {
+  if (p.vec2s) {
+    // C2R, tile order, e stride-1 and whole tiles (the planner's condition):
+    // entry q is element k of lines j = 2 jp and j + 1, adjacent in memory,
+    // written by one store; the launcher checked the pair alignment
+    T *out = reinterpret_cast<T *>(p.out);
+    const T *bufr = reinterpret_cast<const T *>(buf);
+    const int lb = p.logB - 1;
+    T y[2 * E];
+    long long off[E];
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+      const int i = tid + q * nth;
+      const int j = 2 * (i & ((1 << lb) - 1)), k = i >> lb;
+      const bool ok = i < (p.B >> 1) * p.nout;
+      const int o = 2 * P3E_PAD(k >> 1) + (k & 1);
+      y[2 * q] = bufr[ok ? 2 * j * p.lp + o : 0];
+      y[2 * q + 1] = bufr[ok ? 2 * (j + 1) * p.lp + o : 0];
+      off[q] = ok ? base + (long long)k * p.os_d + j : -1;
+    }
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+      if (off[q] < 0) continue;
+      RealPair<T> v;
+      v.a = p.has_scale ? y[2 * q] * (T)p.scale : y[2 * q];
+      v.b = p.has_scale ? y[2 * q + 1] * (T)p.scale : y[2 * q + 1];
+      *reinterpret_cast<RealPair<T> *>(out + off[q]) = v;
+    }
+  } else if (p.cls == CLS_C2R) {
     T *out = reinterpret_cast<T *>(p.out);
     const T *bufr = reinterpret_cast<const T *>(buf);
     // 2E reals per thread, 2CH at a time: LDS reads first, then the global

@@ -267,10 +267,13 @@ extern "C" int p3e_launch_pass(const p3e::PassArgs *a, int prec, size_t lds_byte
   if (nblocks == 0) return 0;
   if (a->fast) {
     if (threads < 1 || threads > P3E_FAST_MAX_THREADS) return (int)hipErrorInvalidValue;
-    if (a->vec2 && (reinterpret_cast<uintptr_t>(a->in) % (2 * (size_t)prec)) != 0) {
-      // the paired loads need the input aligned to two reals
+    const bool in_odd = a->vec2 && (reinterpret_cast<uintptr_t>(a->in) % (2 * (size_t)prec)) != 0;
+    const bool out_odd = a->vec2s && (reinterpret_cast<uintptr_t>(a->out) % (2 * (size_t)prec)) != 0;
+    if (in_odd || out_odd) {
+      // the paired loads / stores need the array aligned to two reals
       p3e::PassArgs b = *a;
-      b.vec2 = 0;
+      if (in_odd) b.vec2 = 0;
+      if (out_odd) b.vec2s = 0;
       return prec == 8 ? p3e::launch_pow2_any<double>(&b, lds_bytes, threads, nblocks, st)
                        : p3e::launch_pow2_any<float>(&b, lds_bytes, threads, nblocks, st);
     }

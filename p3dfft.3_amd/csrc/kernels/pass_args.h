@@ -78,6 +78,7 @@ struct PassArgs {
   int tw_in_lds;     // generic path: the standalone kernel copies tw to LDS
   int ntw;           // entries of the twiddle table tw
   int vec2;          // power-of-two R2C, tile-order load: one load reads lines j, j+1 (fft_fast.h)
+  int vec2s;         // power-of-two C2R, tile-order store: one store writes lines j, j+1
   int mid;           // single-buffer mixed-radix path (fft_mid.h)
   int mid_prime;     // its odd prime radix 7..47 (0: none); selects the kernel instantiation
   int pair;          // R2C/C2R on the single-buffer path: lines 2t, 2t+1 share LDS line t

@@ -299,9 +299,10 @@ def test_r2c_c2r_paired_lines(dims, mo2, prec):
                                           ([32, 8, 48], [1, 2, 0], [1, 2, 0])])
 def test_r2c_tile_load_real_pairs(dims, mo1, mo2, prec):
     """Power-of-two R2C with a tile-order load along a stride-1 axis reads two
-    adjacent lines' reals per load when every tile is whole and the strides
-    are even (the first two cases); the third has a partial last tile and
-    loads one real at a time."""
+    adjacent lines' reals per load, and C2R with a tile-order store writes
+    two per store, when every tile is whole and the strides are even (the
+    first two cases); the third has a partial last tile and moves one real
+    at a time."""
     fwd = [C.R2C_D, C.CF_D, C.CF_D]
     t = fwd if prec == 8 else [C.R2C_S, C.CF_S, C.CF_S]
     x = C.random_input(fwd, dims, mo1, 3)
@@ -309,3 +310,10 @@ def test_r2c_tile_load_real_pairs(dims, mo1, mo2, prec):
     got, desc = emu.exec3d(t, dims, mo1, od, mo2, x if prec == 8 else x.astype(np.float32), prec)
     assert "pow2" in desc.split(" | ")[0], desc
     assert C.rel_l2(got, orc.exec3d(fwd, dims, mo1, mo2, x)) < (1e-12 if prec == 8 else 1e-5), desc
+    # and back: the C2R pass's tile-order store writes two lines' reals per store
+    bwd = [C.C2R_D, C.CB_D, C.CB_D]
+    tb = bwd if prec == 8 else [C.C2R_S, C.CB_S, C.CB_S]
+    z = C.random_input(bwd, od, mo2, 4)
+    got, desc = emu.exec3d(tb, od, mo2, dims, mo1, z if prec == 8 else z.astype(np.complex64), prec)
+    assert "pow2" in desc.split(" | ")[-1], desc
+    assert C.rel_l2(got, orc.exec3d(bwd, od, mo2, mo1, z)) < (1e-12 if prec == 8 else 1e-5), desc
