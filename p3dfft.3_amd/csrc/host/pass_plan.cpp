@@ -297,14 +297,15 @@ static void setup_fast(int prec, const OpSpec &op, PassDesc &ps, int batch_threa
   a.mid = 0;
   a.mid_prime = 0;
   a.pair = 0;
-  // R2C with a tile-order load along a stride-1 e: two adjacent lines' reals
-  // per load when every tile is whole and the strides keep pairs aligned
-  // (the launcher also checks the array's address); standalone passes only
-  a.vec2 = batch_threads == 0 && op.cls == CLS_R2C && a.load_order == ORD_TILE && a.is_e == 1 && B >= 2 &&
-           a.ne % B == 0 && a.is_d % 2 == 0 && a.is_f % 2 == 0;
-  // and the mirror: C2R with a tile-order store along a stride-1 e
-  a.vec2s = batch_threads == 0 && op.cls == CLS_C2R && a.store_order == ORD_TILE && a.os_e == 1 && B >= 2 &&
-            a.ne % B == 0 && a.os_d % 2 == 0 && a.os_f % 2 == 0;
+  // R2C or C2C with a tile-order load along a stride-1 e: two adjacent
+  // lines' elements per load when every tile is whole and the strides keep
+  // pairs aligned (the launcher also checks the array's address);
+  // standalone passes only
+  a.vec2 = batch_threads == 0 && (op.cls == CLS_R2C || op.cls == CLS_C2C) && a.load_order == ORD_TILE &&
+           a.is_e == 1 && B >= 2 && a.ne % B == 0 && a.is_d % 2 == 0 && a.is_f % 2 == 0;
+  // and the mirror for C2R and C2C with a tile-order store along a stride-1 e
+  a.vec2s = batch_threads == 0 && (op.cls == CLS_C2R || op.cls == CLS_C2C) && a.store_order == ORD_TILE &&
+            a.os_e == 1 && B >= 2 && a.ne % B == 0 && a.os_d % 2 == 0 && a.os_f % 2 == 0;
   a.tw_in_lds = 0;
   a.M = M;
   a.logM = 0;

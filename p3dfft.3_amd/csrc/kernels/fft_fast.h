@@ -173,6 +173,10 @@ P3E_HD P3E_INL void c2r_pair_put(const PassArgs &p, cx<T> *line, int k, cx<T> xk
 template <typename T> struct alignas(2 * sizeof(T)) RealPair {
   T a, b;
 };
+// Two complex values in one load (16 or 32 bytes, aligned to 16).
+template <typename T> struct alignas(16) CxPair {
+  T ar, ai, br, bi;
+};
 
 template <typename T, int M>
 P3E_HD P3E_INL void fast_fetch(const PassArgs &p, uint32_t f, int e0, int nb, TileRegs<T> &R, int tid, int nth) {
@@ -181,7 +185,24 @@ P3E_HD P3E_INL void fast_fetch(const PassArgs &p, uint32_t f, int e0, int nb, Ti
   const int tot = (p.load_order == ORD_LINE ? nb : p.B) * p.nin;
   // out-of-tile entries load the tile's first element (always valid) and
   // are not written: unconditional loads keep the values in registers
-  if (p.vec2) {
+  if (p.vec2 && p.cls == CLS_C2C) {
+    // C2C, tile order, e stride-1 and whole tiles: entry q is element n of
+    // lines j = 2 jp and j + 1, adjacent in memory, read by one load
+    const cx<T> *in = reinterpret_cast<const cx<T> *>(p.in);
+    const int lb = p.logB - 1;
+#pragma unroll
+    for (int q = 0; q < E / 2; q++) {
+      const int i = tid + q * nth;
+      const int jp = i & ((1 << lb) - 1), n = i >> lb;
+      const bool ok = i < (p.B >> 1) * p.nin;
+      const CxPair<T> x =
+          *reinterpret_cast<const CxPair<T> *>(in + (ok ? base + (long long)n * p.is_d + 2 * jp : base));
+      R.v[4 * q] = x.ar;
+      R.v[4 * q + 1] = x.ai;
+      R.v[4 * q + 2] = x.br;
+      R.v[4 * q + 3] = x.bi;
+    }
+  } else if (p.vec2) {
     // R2C, tile order, e stride-1 and whole tiles (the planner's condition):
     // entry q is element n of lines j = 2 jp and j + 1, adjacent in memory;
     // the launcher checked that base and strides keep the pair aligned
@@ -245,7 +266,18 @@ P3E_HD P3E_INL void fast_put(const PassArgs &p, uint32_t f, int e0, int nb, cons
                              int tid, int nth) {
   constexpr int E = P3E_FAST_E;
   const int tot = (p.load_order == ORD_LINE ? nb : p.B) * p.nin;
-  if (p.vec2) {
+  if (p.vec2 && p.cls == CLS_C2C) {
+    const int lb = p.logB - 1;
+#pragma unroll
+    for (int q = 0; q < E / 2; q++) {
+      const int i = tid + q * nth;
+      const int j = 2 * (i & ((1 << lb) - 1)), n = i >> lb;
+      if (i < (p.B >> 1) * p.nin) {
+        buf[j * p.lp + P3E_PAD(n)] = mk(R.v[4 * q], R.v[4 * q + 1]);
+        buf[(j + 1) * p.lp + P3E_PAD(n)] = mk(R.v[4 * q + 2], R.v[4 * q + 3]);
+      }
+    }
+  } else if (p.vec2) {
     T *bufr = reinterpret_cast<T *>(buf);
     const int lb = p.logB - 1;
 #pragma unroll
@@ -313,7 +345,7 @@ P3E_HD void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, const cx<T
   const long long base = (long long)e0 * p.os_e + (long long)f * p.os_f;
   const int tot = (p.store_order == ORD_LINE ? nb : p.B) * p.nout;
   constexpr int E = P3E_FAST_E;
-  if (p.vec2s) {
+  if (p.vec2s && p.cls == CLS_C2R) {
     // C2R, tile order, e stride-1 and whole tiles (the planner's condition):
     // entry q is element k of lines j = 2 jp and j + 1, adjacent in memory,
     // written by one store; the launcher checked the pair alignment
@@ -368,6 +400,35 @@ P3E_HD void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, const cx<T
 #pragma unroll
     for (int q = 0; q < 2 * CH; q++)
       if (off[q] >= 0) out[off[q]] = p.has_scale ? y[q] * (T)p.scale : y[q];
+    }
+  } else if (p.vec2s) {
+    // C2C, tile order, e stride-1 and whole tiles: output element k of lines
+    // j = 2 jp and j + 1, adjacent in memory, written by one store
+    cx<T> *out = reinterpret_cast<cx<T> *>(p.out);
+    const int lb = p.logB - 1;
+    const int ptot = (p.B >> 1) * p.nout;
+#pragma unroll 4
+    for (int i = tid; i < ptot; i += nth) {
+      const int j = 2 * (i & ((1 << lb) - 1)), k = i >> lb;
+      cx<T> y[2] = {buf[j * p.lp + P3E_PAD(k)], buf[(j + 1) * p.lp + P3E_PAD(k)]};
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        if (p.deriv) {
+          T kk;
+          if (k < p.dmid) kk = (T)k;
+          else if (k == p.dmid) kk = (T)0;
+          else kk = (T)(k - p.dg);
+          y[h] = mk(-kk * y[h].y, kk * y[h].x);
+        }
+        if (p.spec_op) y[h] = scl(y[h], poisson_factor<T>(p, k, e0 + j + h, (int)f));
+        if (p.has_scale) y[h] = scl(y[h], (T)p.scale);
+      }
+      CxPair<T> v;
+      v.ar = y[0].x;
+      v.ai = y[0].y;
+      v.br = y[1].x;
+      v.bi = y[1].y;
+      *reinterpret_cast<CxPair<T> *>(out + base + (long long)k * p.os_d + j) = v;
     }
   } else {
     cx<T> *out = reinterpret_cast<cx<T> *>(p.out);

@@ -267,10 +267,12 @@ extern "C" int p3e_launch_pass(const p3e::PassArgs *a, int prec, size_t lds_byte
   if (nblocks == 0) return 0;
   if (a->fast) {
     if (threads < 1 || threads > P3E_FAST_MAX_THREADS) return (int)hipErrorInvalidValue;
-    const bool in_odd = a->vec2 && (reinterpret_cast<uintptr_t>(a->in) % (2 * (size_t)prec)) != 0;
-    const bool out_odd = a->vec2s && (reinterpret_cast<uintptr_t>(a->out) % (2 * (size_t)prec)) != 0;
+    // a pair is two reals (R2C in, C2R out) or two complex values (C2C)
+    const size_t pair = (a->cls == p3e::CLS_C2C ? 4 : 2) * (size_t)prec;
+    const bool in_odd = a->vec2 && (reinterpret_cast<uintptr_t>(a->in) % pair) != 0;
+    const bool out_odd = a->vec2s && (reinterpret_cast<uintptr_t>(a->out) % pair) != 0;
     if (in_odd || out_odd) {
-      // the paired loads / stores need the array aligned to two reals
+      // the paired loads / stores need the array aligned to one pair
       p3e::PassArgs b = *a;
       if (in_odd) b.vec2 = 0;
       if (out_odd) b.vec2s = 0;
