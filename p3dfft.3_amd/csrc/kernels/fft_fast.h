@@ -186,17 +186,21 @@ P3E_HD P3E_INL void fast_fetch(const PassArgs &p, uint32_t f, int e0, int nb, Ti
   // out-of-tile entries load the tile's first element (always valid) and
   // are not written: unconditional loads keep the values in registers
   if (p.vec2 && p.cls == CLS_C2C) {
-    // C2C, tile order, e stride-1 and whole tiles: entry q is element n of
-    // lines j = 2 jp and j + 1, adjacent in memory, read by one load
+    // C2C, two adjacent elements per load: in tile order (e stride-1, whole
+    // tiles) element n of lines j = 2 jp and j + 1; in line order (d
+    // stride-1) elements n = 2 np and n + 1 of line j
     const cx<T> *in = reinterpret_cast<const cx<T> *>(p.in);
-    const int lb = p.logB - 1;
+    const bool lo = p.load_order == ORD_LINE;
+    const int lb = lo ? Log2<M>::v - 1 : p.logB - 1;
+    const int ptot = lo ? nb * (M / 2) : (p.B >> 1) * p.nin;
 #pragma unroll
     for (int q = 0; q < E / 2; q++) {
       const int i = tid + q * nth;
-      const int jp = i & ((1 << lb) - 1), n = i >> lb;
-      const bool ok = i < (p.B >> 1) * p.nin;
-      const CxPair<T> x =
-          *reinterpret_cast<const CxPair<T> *>(in + (ok ? base + (long long)n * p.is_d + 2 * jp : base));
+      const int lo_i = 2 * (i & ((1 << lb) - 1)), hi_i = i >> lb;
+      const int j = lo ? hi_i : lo_i, n = lo ? lo_i : hi_i;
+      const bool ok = i < ptot;
+      const CxPair<T> x = *reinterpret_cast<const CxPair<T> *>(
+          in + (ok ? base + (long long)n * p.is_d + (long long)j * p.is_e : base));
       R.v[4 * q] = x.ar;
       R.v[4 * q + 1] = x.ai;
       R.v[4 * q + 2] = x.br;
@@ -267,14 +271,18 @@ P3E_HD P3E_INL void fast_put(const PassArgs &p, uint32_t f, int e0, int nb, cons
   constexpr int E = P3E_FAST_E;
   const int tot = (p.load_order == ORD_LINE ? nb : p.B) * p.nin;
   if (p.vec2 && p.cls == CLS_C2C) {
-    const int lb = p.logB - 1;
+    const bool lo = p.load_order == ORD_LINE;
+    const int lb = lo ? Log2<M>::v - 1 : p.logB - 1;
+    const int ptot = lo ? nb * (M / 2) : (p.B >> 1) * p.nin;
 #pragma unroll
     for (int q = 0; q < E / 2; q++) {
       const int i = tid + q * nth;
-      const int j = 2 * (i & ((1 << lb) - 1)), n = i >> lb;
-      if (i < (p.B >> 1) * p.nin) {
+      const int lo_i = 2 * (i & ((1 << lb) - 1)), hi_i = i >> lb;
+      const int j = lo ? hi_i : lo_i, n = lo ? lo_i : hi_i;
+      if (i < ptot) {
+        // the second element: line j + 1 (tile order) or element n + 1 (line order)
         buf[j * p.lp + P3E_PAD(n)] = mk(R.v[4 * q], R.v[4 * q + 1]);
-        buf[(j + 1) * p.lp + P3E_PAD(n)] = mk(R.v[4 * q + 2], R.v[4 * q + 3]);
+        buf[lo ? j * p.lp + P3E_PAD(n + 1) : (j + 1) * p.lp + P3E_PAD(n)] = mk(R.v[4 * q + 2], R.v[4 * q + 3]);
       }
     }
   } else if (p.vec2) {
@@ -402,25 +410,30 @@ P3E_HD void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, const cx<T
       if (off[q] >= 0) out[off[q]] = p.has_scale ? y[q] * (T)p.scale : y[q];
     }
   } else if (p.vec2s) {
-    // C2C, tile order, e stride-1 and whole tiles: output element k of lines
-    // j = 2 jp and j + 1, adjacent in memory, written by one store
+    // C2C, two adjacent output elements per store: in tile order (e
+    // stride-1, whole tiles) element k of lines j = 2 jp and j + 1; in line
+    // order (d stride-1) elements k = 2 kp and k + 1 of line j
     cx<T> *out = reinterpret_cast<cx<T> *>(p.out);
-    const int lb = p.logB - 1;
-    const int ptot = (p.B >> 1) * p.nout;
+    const bool lo = p.store_order == ORD_LINE;
+    const int lb = lo ? Log2<M>::v - 1 : p.logB - 1;
+    const int ptot = lo ? nb * (M / 2) : (p.B >> 1) * p.nout;
 #pragma unroll 4
     for (int i = tid; i < ptot; i += nth) {
-      const int j = 2 * (i & ((1 << lb) - 1)), k = i >> lb;
-      cx<T> y[2] = {buf[j * p.lp + P3E_PAD(k)], buf[(j + 1) * p.lp + P3E_PAD(k)]};
+      const int lo_i = 2 * (i & ((1 << lb) - 1)), hi_i = i >> lb;
+      const int j = lo ? hi_i : lo_i, k = lo ? lo_i : hi_i;
+      const int j1 = lo ? j : j + 1, k1 = lo ? k + 1 : k;  // the second element
+      cx<T> y[2] = {buf[j * p.lp + P3E_PAD(k)], buf[j1 * p.lp + P3E_PAD(k1)]};
 #pragma unroll
       for (int h = 0; h < 2; h++) {
+        const int kh = h ? k1 : k, jh = h ? j1 : j;
         if (p.deriv) {
           T kk;
-          if (k < p.dmid) kk = (T)k;
-          else if (k == p.dmid) kk = (T)0;
-          else kk = (T)(k - p.dg);
+          if (kh < p.dmid) kk = (T)kh;
+          else if (kh == p.dmid) kk = (T)0;
+          else kk = (T)(kh - p.dg);
           y[h] = mk(-kk * y[h].y, kk * y[h].x);
         }
-        if (p.spec_op) y[h] = scl(y[h], poisson_factor<T>(p, k, e0 + j + h, (int)f));
+        if (p.spec_op) y[h] = scl(y[h], poisson_factor<T>(p, kh, e0 + jh, (int)f));
         if (p.has_scale) y[h] = scl(y[h], (T)p.scale);
       }
       CxPair<T> v;
@@ -428,7 +441,7 @@ P3E_HD void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, const cx<T
       v.ai = y[0].y;
       v.br = y[1].x;
       v.bi = y[1].y;
-      *reinterpret_cast<CxPair<T> *>(out + base + (long long)k * p.os_d + j) = v;
+      *reinterpret_cast<CxPair<T> *>(out + base + (long long)k * p.os_d + (long long)j * p.os_e) = v;
     }
   } else {
     cx<T> *out = reinterpret_cast<cx<T> *>(p.out);
