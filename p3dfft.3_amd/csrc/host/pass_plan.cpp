@@ -303,9 +303,9 @@ static void setup_fast(int prec, const OpSpec &op, PassDesc &ps, int batch_threa
   // standalone passes only
   a.vec2 = batch_threads == 0 && (op.cls == CLS_R2C || op.cls == CLS_C2C) && a.load_order == ORD_TILE &&
            a.is_e == 1 && B >= 2 && a.ne % B == 0 && a.is_d % 2 == 0 && a.is_f % 2 == 0;
-  // and the mirror for C2R and C2C with a tile-order store along a stride-1 e
-  a.vec2s = batch_threads == 0 && (op.cls == CLS_C2R || op.cls == CLS_C2C) && a.store_order == ORD_TILE &&
-            a.os_e == 1 && B >= 2 && a.ne % B == 0 && a.os_d % 2 == 0 && a.os_f % 2 == 0;
+  // and the mirror for all three with a tile-order store along a stride-1 e
+  a.vec2s = batch_threads == 0 && a.store_order == ORD_TILE && a.os_e == 1 && B >= 2 && a.ne % B == 0 &&
+            a.os_d % 2 == 0 && a.os_f % 2 == 0;
   // C2C with a line-order side (d stride-1): two adjacent elements of a line
   // per load / store when the line strides keep pairs aligned
   if (batch_threads == 0 && op.cls == CLS_C2C && a.load_order == ORD_LINE && a.is_d == 1 && a.is_e % 2 == 0 &&

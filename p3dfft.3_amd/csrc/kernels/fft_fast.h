@@ -410,9 +410,9 @@ P3E_HD void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, const cx<T
       if (off[q] >= 0) out[off[q]] = p.has_scale ? y[q] * (T)p.scale : y[q];
     }
   } else if (p.vec2s) {
-    // C2C, two adjacent output elements per store: in tile order (e
+    // C2C and R2C, two adjacent output elements per store: in tile order (e
     // stride-1, whole tiles) element k of lines j = 2 jp and j + 1; in line
-    // order (d stride-1) elements k = 2 kp and k + 1 of line j
+    // order (C2C only, d stride-1) elements k = 2 kp and k + 1 of line j
     cx<T> *out = reinterpret_cast<cx<T> *>(p.out);
     const bool lo = p.store_order == ORD_LINE;
     const int lb = lo ? Log2<M>::v - 1 : p.logB - 1;
@@ -422,7 +422,21 @@ P3E_HD void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, const cx<T
       const int lo_i = 2 * (i & ((1 << lb) - 1)), hi_i = i >> lb;
       const int j = lo ? hi_i : lo_i, k = lo ? lo_i : hi_i;
       const int j1 = lo ? j : j + 1, k1 = lo ? k + 1 : k;  // the second element
-      cx<T> y[2] = {buf[j * p.lp + P3E_PAD(k)], buf[j1 * p.lp + P3E_PAD(k1)]};
+      cx<T> y[2];
+      if (p.cls == CLS_R2C) {
+        // X_k = (Z_k + conj Z_{M-k})/2 - i w_N^k (Z_k - conj Z_{M-k})/2, per line
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+          const cx<T> *l = buf + (h ? j1 : j) * p.lp;
+          const cx<T> zk = l[P3E_PAD(k & (M - 1))];
+          const cx<T> zm = l[P3E_PAD((M - k) & (M - 1))];
+          const cx<T> e = add(zk, conj(zm)), o = sub(zk, conj(zm));
+          y[h] = scl(add(e, rot(mul(wN[k], o), -1)), (T)0.5);
+        }
+      } else {
+        y[0] = buf[j * p.lp + P3E_PAD(k)];
+        y[1] = buf[j1 * p.lp + P3E_PAD(k1)];
+      }
 #pragma unroll
       for (int h = 0; h < 2; h++) {
         const int kh = h ? k1 : k, jh = h ? j1 : j;

@@ -267,10 +267,11 @@ extern "C" int p3e_launch_pass(const p3e::PassArgs *a, int prec, size_t lds_byte
   if (nblocks == 0) return 0;
   if (a->fast) {
     if (threads < 1 || threads > P3E_FAST_MAX_THREADS) return (int)hipErrorInvalidValue;
-    // a pair is two reals (R2C in, C2R out) or two complex values (C2C)
-    const size_t pair = (a->cls == p3e::CLS_C2C ? 4 : 2) * (size_t)prec;
-    const bool in_odd = a->vec2 && (reinterpret_cast<uintptr_t>(a->in) % pair) != 0;
-    const bool out_odd = a->vec2s && (reinterpret_cast<uintptr_t>(a->out) % pair) != 0;
+    // a pair is two reals (R2C in, C2R out) or two complex values
+    const size_t in_pair = (a->cls == p3e::CLS_R2C ? 2 : 4) * (size_t)prec;
+    const size_t out_pair = (a->cls == p3e::CLS_C2R ? 2 : 4) * (size_t)prec;
+    const bool in_odd = a->vec2 && (reinterpret_cast<uintptr_t>(a->in) % in_pair) != 0;
+    const bool out_odd = a->vec2s && (reinterpret_cast<uintptr_t>(a->out) % out_pair) != 0;
     if (in_odd || out_odd) {
       // the paired loads / stores need the array aligned to one pair
       p3e::PassArgs b = *a;
