@@ -306,6 +306,14 @@ static void setup_fast(int prec, const OpSpec &op, PassDesc &ps, int batch_threa
   // and the mirror for all three with a tile-order store along a stride-1 e
   a.vec2s = batch_threads == 0 && a.store_order == ORD_TILE && a.os_e == 1 && B >= 2 && a.ne % B == 0 &&
             a.os_d % 2 == 0 && a.os_f % 2 == 0;
+  // R2C loads and C2R stores in line order (d stride-1): a real pair is one
+  // packed complex element; aligned when the line strides are even
+  if (batch_threads == 0 && op.cls == CLS_R2C && a.load_order == ORD_LINE && a.is_d == 1 && a.is_e % 2 == 0 &&
+      a.is_f % 2 == 0)
+    a.vec2 = 1;
+  if (batch_threads == 0 && op.cls == CLS_C2R && a.store_order == ORD_LINE && a.os_d == 1 && a.os_e % 2 == 0 &&
+      a.os_f % 2 == 0)
+    a.vec2s = 1;
   // C2C with a line-order side (d stride-1): two adjacent elements of a line
   // per load / store when the line strides keep pairs aligned
   if (batch_threads == 0 && op.cls == CLS_C2C && a.load_order == ORD_LINE && a.is_d == 1 && a.is_e % 2 == 0 &&

@@ -206,6 +206,21 @@ P3E_HD P3E_INL void fast_fetch(const PassArgs &p, uint32_t f, int e0, int nb, Ti
       R.v[4 * q + 2] = x.br;
       R.v[4 * q + 3] = x.bi;
     }
+  } else if (p.vec2 && p.load_order == ORD_LINE) {
+    // R2C, line order (d stride-1): entry q is the real pair (x_2m, x_2m+1)
+    // of line j, the packed complex element m, read by one load
+    const T *in = reinterpret_cast<const T *>(p.in);
+    constexpr int logM = Log2<M>::v;
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+      const int i = tid + q * nth;
+      const int j = i >> logM, m = i & (M - 1);
+      const bool ok = j < nb;
+      const RealPair<T> x =
+          *reinterpret_cast<const RealPair<T> *>(in + (ok ? base + 2 * m + (long long)j * p.is_e : base));
+      R.v[2 * q] = x.a;
+      R.v[2 * q + 1] = x.b;
+    }
   } else if (p.vec2) {
     // R2C, tile order, e stride-1 and whole tiles (the planner's condition):
     // entry q is element n of lines j = 2 jp and j + 1, adjacent in memory;
@@ -285,6 +300,14 @@ P3E_HD P3E_INL void fast_put(const PassArgs &p, uint32_t f, int e0, int nb, cons
         buf[lo ? j * p.lp + P3E_PAD(n + 1) : (j + 1) * p.lp + P3E_PAD(n)] = mk(R.v[4 * q + 2], R.v[4 * q + 3]);
       }
     }
+  } else if (p.vec2 && p.load_order == ORD_LINE) {
+    constexpr int logM = Log2<M>::v;
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+      const int i = tid + q * nth;
+      const int j = i >> logM, m = i & (M - 1);
+      if (j < nb) buf[j * p.lp + P3E_PAD(m)] = mk(R.v[2 * q], R.v[2 * q + 1]);
+    }
   } else if (p.vec2) {
     T *bufr = reinterpret_cast<T *>(buf);
     const int lb = p.logB - 1;
@@ -353,7 +376,20 @@ P3E_HD void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, const cx<T
   const long long base = (long long)e0 * p.os_e + (long long)f * p.os_f;
   const int tot = (p.store_order == ORD_LINE ? nb : p.B) * p.nout;
   constexpr int E = P3E_FAST_E;
-  if (p.vec2s && p.cls == CLS_C2R) {
+  if (p.vec2s && p.cls == CLS_C2R && p.store_order == ORD_LINE) {
+    // C2R, line order (d stride-1): the packed complex element m of line j
+    // is the real pair (x_2m, x_2m+1), written by one store
+    T *out = reinterpret_cast<T *>(p.out);
+#pragma unroll 4
+    for (int i = tid; i < nb * M; i += nth) {
+      const int j = i >> logM, m = i & (M - 1);
+      const cx<T> z = buf[j * p.lp + P3E_PAD(m)];
+      RealPair<T> v;
+      v.a = p.has_scale ? z.x * (T)p.scale : z.x;
+      v.b = p.has_scale ? z.y * (T)p.scale : z.y;
+      *reinterpret_cast<RealPair<T> *>(out + base + 2 * m + (long long)j * p.os_e) = v;
+    }
+  } else if (p.vec2s && p.cls == CLS_C2R) {
     // C2R, tile order, e stride-1 and whole tiles (the planner's condition):
     // entry q is element k of lines j = 2 jp and j + 1, adjacent in memory,
     // written by one store; the launcher checked the pair alignment
