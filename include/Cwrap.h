@@ -29,7 +29,12 @@
  * forward sign is -1.  Differences: exec returns after the result is in
  * `out` (synchronous), arrays may be device or host pointers, and errors
  * are also recorded for p3dfft_hip_last_error() (p3dfft_hip.h).
- * The Fortran *_f entry points (Cwrap.h:123-134) are not provided.
+ * The Fortran-side *_f entry points are declared in Fwrap.h.
+ *
+ * Ordering: the synchronous calls run on the library's own stream after
+ * everything already queued on the null (legacy default) stream, so a
+ * kernel that fills `in` there needs no synchronization first; work on
+ * other streams must be complete (or ordered by the caller) before the call.
  */
 #ifndef P3DFFT_CWRAP_H
 #define P3DFFT_CWRAP_H
@@ -70,6 +75,10 @@ void p3dfft_cleanup();
 Type3D p3dfft_init_3Dtype(int[3]);
 int p3dfft_plan_1Dtrans(Grid *, Grid *, int, int);
 Plan3D p3dfft_plan_3Dtrans(Grid *, Grid *, Type3D);
+/* Cwrap.h:106, build/wrap.C:279-294: handle of the data grid made by
+ * p3dfft_init_data_grid_f with the same global dims, processor grid, Dmap
+ * and MemOrder, or -1. */
+int find_grid(int[3], int, int *, int[3]);
 int p3dfft_init_proc_grid(int pdims[3], MPI_Comm comm);
 Grid *p3dfft_init_data_grid(int[3], int, int, int[3], int[3]);
 void p3dfft_free_data_grid(Grid *gr);

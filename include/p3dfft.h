@@ -26,6 +26,7 @@
  *   gen_transform3D             :869-875
  *   transform3D                 :877-896
  *   compute_deriv               :917
+ *   timer / timers (TIMERS)     :958-992
  * MPIplan / trans_MPIplan (:549-577, :625-651) are multi-rank exchange
  * stages, never built on a 1x1 grid, and are not provided.
  */
@@ -241,6 +242,37 @@ bool find_order(int L[3], const trans_type3D *tp, const DataGrid *gr1, const Dat
 
 template <class Type> void compute_deriv(Type *in, Type *out, DataGrid *gr, int idir);
 
+#ifdef TIMERS
+// include/p3dfft.h:958-992 of the reference: accumulated seconds per phase of
+// the synchronous exec calls since init(); print() writes the reference's
+// "TIMERS (avg/min/max)" report on rank 0 of comm.  The library is built
+// with the class in any case; programs see it when they define TIMERS, as
+// with the reference.  On a 1x1 grid the exchange phases (packsend,
+// packsend_trans, unpackrecv, alltoall) stay 0, and every fused reorder +
+// transform pass counts as reorder_trans (trans_exec when it keeps the
+// memory order), timed on the GPU with HIP events.
+class timer {
+ protected:
+  double reorder_deriv;
+  double reorder_trans;
+  double reorder_out;
+  double reorder_in;
+  double trans_exec;
+  double trans_deriv;
+  double packsend;
+  double packsend_trans;
+  double packsend_deriv;
+  double unpackrecv;
+  double alltoall;
+
+ public:
+  void init();
+  void print(MPI_Comm);
+};
+
+extern timer timers;
+#endif
+
 extern vector<gen_trans_type *> types1D;
 extern vector<trans_type3D> types3D;
 extern vector<gen_transform3D *> stored_trans3D;
@@ -254,6 +286,7 @@ extern "C" {
 #endif /* __cplusplus */
 
 #include "Cwrap.h"
+#include "Fwrap.h"
 
 #ifdef __cplusplus
 }

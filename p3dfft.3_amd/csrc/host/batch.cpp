@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <string>
 
 extern "C" int p3e_launch_step(const void *table, const void *work, int nwork, int prec, size_t lds,
@@ -61,15 +62,16 @@ bool upload_pass_tables(int prec, PassDesc &pd, std::vector<void *> &owned) {
 }
 
 unsigned int persistent_grid(int prec, size_t lds, int nwork) {
-  static int cus = 0;
-  if (cus == 0) {
-    int dev = 0;
-    cus = 256;
-    if (hipGetDevice(&dev) == hipSuccess) {
-      int n = 0;
-      if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0) cus = n;
-    }
-  }
+  // CU count of the current device, looked up once (the batch and the
+  // Poisson service call this from different threads)
+  static std::once_flag once;
+  static int cus = 256;
+  std::call_once(once, [] {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+      cus = n;
+  });
   int per_cu = p3e_step_occupancy(prec, lds);
   if (per_cu < 1) per_cu = 1;
   long g = (long)cus * per_cu;

@@ -15,6 +15,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 
 extern "C" int p3e_launch_pass(const p3e::PassArgs *a, int prec, size_t lds_bytes, int threads,
                                unsigned int nblocks, hipStream_t st);
@@ -41,11 +42,41 @@ hipStream_t default_stream() {
   return g_stream;
 }
 
+// The synchronous entry points run on the library's stream, which is
+// non-blocking: without an explicit dependency it would not wait for work
+// the caller queued on the null stream (e.g. a kernel that fills `in`).
+// An event recorded on the null stream and waited for on `st` orders the
+// exec after everything queued there before the call.
+static thread_local hipEvent_t t_null_ev = nullptr;
+
+int order_after_null_stream(hipStream_t st) {
+  if (!t_null_ev && hipEventCreateWithFlags(&t_null_ev, hipEventDisableTiming) != hipSuccess) {
+    t_null_ev = nullptr;
+    set_error("hipEventCreateWithFlags failed");
+    return -1;
+  }
+  HIPCHK(hipEventRecord(t_null_ev, nullptr));
+  HIPCHK(hipStreamWaitEvent(st, t_null_ev, 0));
+  return 0;
+}
+
+// Device copies of host arrays passed to compute_deriv, kept across calls
+// (grown when a larger array comes).
+static std::mutex g_tmp_mu;
+static void *g_tmp[2] = {nullptr, nullptr};
+static size_t g_tmp_cap[2] = {0, 0};
+
 void release_all() {
   if (g_stream) {
     (void)hipStreamSynchronize(g_stream);
     (void)hipStreamDestroy(g_stream);
     g_stream = nullptr;
+  }
+  std::lock_guard<std::mutex> g(g_tmp_mu);
+  for (int i = 0; i < 2; i++) {
+    if (g_tmp[i]) (void)hipFree(g_tmp[i]);
+    g_tmp[i] = nullptr;
+    g_tmp_cap[i] = 0;
   }
 }
 
@@ -130,6 +161,7 @@ Plan *plan_from_chain(int prec, PassChain &chain, const Layout &in, const Layout
 void plan_destroy(Plan *p) {
   if (!p) return;
   for (hipEvent_t e : p->ev_pool) (void)hipEventDestroy(e);
+  for (hipEvent_t e : p->tm_ev) (void)hipEventDestroy(e);
   for (Pass &ps : p->passes) free_tables(ps);
   for (int i = 0; i < 2; i++)
     if (p->scratch[i]) (void)hipFree(p->scratch[i]);
@@ -169,6 +201,7 @@ int plan_exec(Plan *p, const void *in, void *out, int idir, bool OW, hipStream_t
     return -1;
   }
   if (!st) st = default_stream();
+  if (sync && order_after_null_stream(st)) return -1;
   const void *din = in;
   void *dout = out;
   const bool in_host = !is_device_ptr(in);
@@ -183,8 +216,24 @@ int plan_exec(Plan *p, const void *in, void *out, int idir, bool OW, hipStream_t
     dout = p->stage_out;
   }
   const bool alias = (din == (const void *)dout);
+  // TIMERS report: events around each pass of a synchronous exec
+  const bool tm = sync && timers_enabled();
+  int tm_phase[8];
+  int tm_n = 0;
+  if (tm) {
+    while (p->tm_ev.size() < 2 * (p->passes.size() + 1)) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      p->tm_ev.push_back(e);
+    }
+  }
   if (p->copy_only) {
+    if (tm) HIPCHK(hipEventRecord(p->tm_ev[0], st));
     if (!alias) HIPCHK(hipMemcpyAsync(dout, din, p->out_bytes, hipMemcpyDeviceToDevice, st));
+    if (tm) {
+      HIPCHK(hipEventRecord(p->tm_ev[1], st));
+      tm_phase[tm_n++] = PH_TRANS_EXEC;  // build/exec.C:546-574: the memcpy of an EMPTY stage
+    }
   } else {
     std::vector<PassDesc> descs;
     for (Pass &ps : p->passes) descs.push_back(ps.d);
@@ -220,7 +269,28 @@ int plan_exec(Plan *p, const void *in, void *out, int idir, bool OW, hipStream_t
         p->ev_pass.push_back((int)i);
         HIPCHK(hipEventRecord(e0, st));
       }
+      int ph = -1;
+      if (tm) {
+        // the reference's regimes (build/exec.C:546-620): a pure reorder is
+        // reorder_out (reorder_in in place); a transform that keeps the
+        // memory order is trans_exec, one that permutes it reorder_trans;
+        // with the derivative, trans_deriv / reorder_deriv
+        const bool same = ps.d.lin.mo[0] == ps.d.lout.mo[0] && ps.d.lin.mo[1] == ps.d.lout.mo[1] &&
+                          ps.d.lin.mo[2] == ps.d.lout.mo[2];
+        if (a.cls == CLS_EMPTY) ph = (a.in == a.out) ? PH_REORDER_IN : PH_REORDER_OUT;
+        else if (a.deriv) ph = same ? PH_TRANS_DERIV : PH_REORDER_DERIV;
+        else ph = same ? PH_TRANS_EXEC : PH_REORDER_TRANS;
+        static const char *names[PH_COUNT] = {"p3dfft reorder_deriv", "p3dfft reorder_trans", "p3dfft reorder_out",
+                                              "p3dfft reorder_in",    "p3dfft trans_exec",    "p3dfft trans_deriv"};
+        timers_range_push(names[ph]);
+        HIPCHK(hipEventRecord(p->tm_ev[2 * tm_n], st));
+      }
       int rc = p3e_launch_pass(&a, p->prec, ps.d.lds_bytes, ps.d.threads, ps.d.nblocks, st);
+      if (tm) {
+        timers_range_pop();
+        HIPCHK(hipEventRecord(p->tm_ev[2 * tm_n + 1], st));
+        tm_phase[tm_n++] = ph;
+      }
       if (p->prof) HIPCHK(hipEventRecord(e1, st));
       if (rc) {
         set_error("pass kernel launch failed: %s", hipGetErrorString((hipError_t)rc));
@@ -232,6 +302,11 @@ int plan_exec(Plan *p, const void *in, void *out, int idir, bool OW, hipStream_t
   }
   if (out_host) HIPCHK(hipMemcpyAsync(out, dout, p->out_bytes, hipMemcpyDeviceToHost, st));
   if (sync || in_host || out_host) HIPCHK(hipStreamSynchronize(st));
+  for (int i = 0; i < tm_n; i++) {
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, p->tm_ev[2 * i], p->tm_ev[2 * i + 1]));
+    timers_add(tm_phase[i], ms * 1e-3);
+  }
   return 0;
 }
 
@@ -265,21 +340,25 @@ int plan_profile_read(Plan *p, double *avg_ms, int cap, double *total_ms) {
 int deriv_exec(int prec, const void *in, void *out, const int sdims[3], int ldir, int g, int mid, int gstart,
                hipStream_t st) {
   if (!st) st = default_stream();
+  if (order_after_null_stream(st)) return -1;
   const long long n = (long long)sdims[0] * sdims[1] * sdims[2];
   const size_t bytes = (size_t)n * 2 * prec;
   const bool in_host = !is_device_ptr(in), out_host = !is_device_ptr(out);
+  std::lock_guard<std::mutex> g_lock(g_tmp_mu);
   const void *di = in;
   void *dout = out;
-  void *tmp_in = nullptr, *tmp_out = nullptr;
   if (in_host) {
-    HIPCHK(hipMalloc(&tmp_in, bytes));
-    HIPCHK(hipMemcpyAsync(tmp_in, in, bytes, hipMemcpyHostToDevice, st));
-    di = tmp_in;
+    if (ensure(&g_tmp[0], &g_tmp_cap[0], std::max(bytes, (size_t)1))) return -1;
+    HIPCHK(hipMemcpyAsync(g_tmp[0], in, bytes, hipMemcpyHostToDevice, st));
+    di = g_tmp[0];
   }
   if (out_host) {
-    if (in_host && in == out) tmp_out = tmp_in;
-    else HIPCHK(hipMalloc(&tmp_out, bytes));
-    dout = tmp_out;
+    if (in_host && in == out) {
+      dout = g_tmp[0];
+    } else {
+      if (ensure(&g_tmp[1], &g_tmp_cap[1], std::max(bytes, (size_t)1))) return -1;
+      dout = g_tmp[1];
+    }
   }
   int rc = p3e_launch_deriv(di, dout, n, sdims[0], sdims[1], ldir, g, mid, gstart, prec, st);
   if (rc) {
@@ -287,9 +366,9 @@ int deriv_exec(int prec, const void *in, void *out, const int sdims[3], int ldir
     return -1;
   }
   if (out_host) HIPCHK(hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, st));
+  // the cached temporaries are reused by the next call: wait here whatever
+  // the arrays are (the reference's compute_deriv is synchronous)
   HIPCHK(hipStreamSynchronize(st));
-  if (tmp_in) (void)hipFree(tmp_in);
-  if (tmp_out && tmp_out != tmp_in) (void)hipFree(tmp_out);
   return 0;
 }
 

@@ -15,6 +15,21 @@ struct Pass {
   std::vector<void *> tables;  // device copies owned by the pass
 };
 
+// Phases of the reference's TIMERS report (fields of p3dfft::timer), which
+// the synchronous exec calls accumulate while the report is enabled.
+enum Phase {
+  PH_REORDER_DERIV = 0, PH_REORDER_TRANS, PH_REORDER_OUT, PH_REORDER_IN, PH_TRANS_EXEC, PH_TRANS_DERIV, PH_COUNT
+};
+void timers_enable(bool on);
+bool timers_enabled();
+void timers_reset();
+void timers_add(int phase, double seconds);
+void timers_read(double seconds[PH_COUNT]);
+// roctx range around a pass launch while the report is enabled (no-ops
+// when libroctx64 is not loadable)
+void timers_range_push(const char *name);
+void timers_range_pop();
+
 struct Plan {
   int prec;  // 4 or 8
   Layout lin, lout;
@@ -32,6 +47,8 @@ struct Plan {
   std::vector<hipEvent_t> ev_pool;
   std::vector<int> ev_pass;  // pass index of each recorded (start, stop) pair
   size_t ev_used;
+  // TIMERS: one (start, stop) pair per pass of the current synchronous exec
+  std::vector<hipEvent_t> tm_ev;
 };
 
 int plan_profile(Plan *p, bool enable);

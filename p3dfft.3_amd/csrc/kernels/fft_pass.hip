@@ -13,6 +13,10 @@
 // they wrap (build/init.C:1146-1188).
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
+#include <utility>
+
 #include "fft_core.h"
 #include "fft_fast.h"
 #include "fft_mid.h"
@@ -316,12 +320,16 @@ extern "C" int p3e_launch_step(const void *table, const void *work, int nwork, i
   return (int)hipGetLastError();
 }
 
-// Workgroups of the step kernel resident per CU for a given dynamic LDS size.
+// Workgroups of the step kernel resident per CU for a given dynamic LDS size
+// (cached per precision and LDS size; the batch and the Poisson service call
+// this from different threads).
 extern "C" int p3e_step_occupancy(int prec, size_t lds) {
-  static size_t last_lds[2] = {(size_t)-1, (size_t)-1};
-  static int last_n[2] = {1, 1};
-  const int k = prec == 8 ? 1 : 0;
-  if (last_lds[k] == lds) return last_n[k];
+  static std::mutex mu;
+  static std::map<std::pair<int, size_t>, int> cache;
+  const int k = prec == 8 ? 8 : 4;
+  std::lock_guard<std::mutex> g(mu);
+  auto it = cache.find({k, lds});
+  if (it != cache.end()) return it->second;
   const void *fn = prec == 8 ? (const void *)p3e::step_kernel<double> : (const void *)p3e::step_kernel<float>;
   if (lds > 65536) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   int n = 0;
@@ -329,8 +337,7 @@ extern "C" int p3e_step_occupancy(int prec, size_t lds) {
     (void)hipGetLastError();
     n = 1;
   }
-  last_lds[k] = lds;
-  last_n[k] = n;
+  cache[{k, lds}] = n;
   return n;
 }
 

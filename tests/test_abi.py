@@ -25,9 +25,10 @@ def declared_globals(header):
 
 def test_library_exports_every_declared_symbol():
     L = P.lib()
-    fns = declared_functions("Cwrap.h") + declared_functions("p3dfft_hip.h")
-    assert len(fns) >= 26
-    for f in fns:
+    fns = declared_functions("Cwrap.h") + declared_functions("Fwrap.h") + declared_functions("p3dfft_hip.h")
+    assert len(fns) >= 40
+    assert len(declared_functions("Fwrap.h")) == 13
+    for f in fns + ["find_grid"]:
         assert hasattr(L, f), f
     gl = declared_globals("Cwrap.h")
     assert len(gl) == 44
@@ -101,3 +102,52 @@ def test_invalid_handles_record_errors():
     assert P.lib().p3dfft_init_3Dtype((ctypes.c_int * 3)(5, 9, 99)) >= 0
     rc, msg = P.last_error()
     assert rc != 0 and "undefined 1D type ID 99" in msg
+
+
+I3 = ctypes.c_int * 3
+
+
+def _grid_f(gdims, conj, pg, mo):
+    h, ld, gs = ctypes.c_int(), I3(), I3()
+    P.lib().p3dfft_init_data_grid_f(ctypes.byref(h), ld, gs, I3(*gdims), ctypes.byref(ctypes.c_int(conj)),
+                                    ctypes.byref(ctypes.c_int(pg)), I3(0, 1, 2), I3(*mo))
+    return h.value, list(ld), list(gs)
+
+
+def test_fortran_side_grid_handles():
+    """p3dfft_init_data_grid_f (build/wrap.C:725-762): int grid handles, Ldims
+    and GlobStart written back, and an equal grid (global dims, processor
+    grid, Dmap, MemOrder; dim_conj_sym is not compared) reuses its handle
+    (find_grid, build/wrap.C:279-294)."""
+    P.setup()
+    L = P.lib()
+    pg = L.p3dfft_init_proc_grid_f(I3(1, 1, 1), ctypes.byref(ctypes.c_int(P.MPI_COMM_WORLD)))
+    assert pg >= 0
+    a, ld, gs = _grid_f([12, 10, 8], -1, pg, [0, 1, 2])
+    assert a >= 0 and ld == [12, 10, 8] and gs == [0, 0, 0]
+    b, _, _ = _grid_f([12, 10, 8], -1, pg, [0, 1, 2])
+    assert b == a
+    c, _, _ = _grid_f([12, 10, 8], 0, pg, [0, 1, 2])
+    assert c == a  # the reference's key leaves dim_conj_sym out
+    d, _, _ = _grid_f([12, 10, 8], -1, pg, [2, 1, 0])
+    assert d != a
+    assert L.find_grid(I3(12, 10, 8), pg, I3(0, 1, 2), I3(2, 1, 0)) == d
+    assert L.find_grid(I3(12, 10, 9), pg, I3(0, 1, 2), I3(2, 1, 0)) == -1
+    t = ctypes.c_int(-5)
+    L.p3dfft_init_3Dtype_f(ctypes.byref(t), I3(P.type_id("R2CFFT_D"), P.type_id("CFFT_FORWARD_D"),
+                                             P.type_id("CFFT_FORWARD_D")))
+    assert t.value >= 0
+
+
+def test_fortran_side_invalid_handles():
+    P.setup()
+    L = P.lib()
+    P.last_error()
+    plan = ctypes.c_int(7)
+    L.p3dfft_plan_3Dtrans_f(ctypes.byref(plan), ctypes.byref(ctypes.c_int(9999)),
+                            ctypes.byref(ctypes.c_int(9999)), ctypes.byref(ctypes.c_int(0)))
+    rc, msg = P.last_error()
+    assert plan.value == -1 and rc != 0 and "invalid data grid handle" in msg
+    L.p3dfft_compute_deriv_double_f(None, None, ctypes.byref(ctypes.c_int(-1)), ctypes.byref(ctypes.c_int(1)))
+    rc, msg = P.last_error()
+    assert rc != 0 and "invalid data grid handle" in msg

@@ -7,6 +7,7 @@ reference's test scripts grep for the same words (extra/TESTING_README).
 These programs pass host arrays, so this exercises the staged host path of
 the library as an unmodified P3DFFT++ program would."""
 import os
+import re
 import subprocess
 import tempfile
 
@@ -44,8 +45,7 @@ CASES = [
 ]
 
 
-@pytest.mark.parametrize("prog,files", CASES, ids=[f"{p}-{i}" for i, (p, _) in enumerate(CASES)])
-def test_reference_sample(prog, files):
+def run_sample(prog, files):
     exe = os.path.join(BIN, prog)
     if not os.path.exists(exe):
         pytest.skip("samples not built (tests/samples/Makefile needs the reference sources)")
@@ -55,7 +55,58 @@ def test_reference_sample(prog, files):
                 f.write(text)
         print("case:", prog, files, flush=True)
         r = subprocess.run([exe], cwd=d, capture_output=True, text=True, timeout=120)
-    out = r.stdout + r.stderr
+    return r, r.stdout + r.stderr
+
+
+@pytest.mark.parametrize("prog,files", CASES, ids=[f"{p}-{i}" for i, (p, _) in enumerate(CASES)])
+def test_reference_sample(prog, files):
+    r, out = run_sample(prog, files)
     assert r.returncode == 0, out[-3000:]
     assert "Results are correct" in out, out[-3000:]
     assert "incorrect" not in out, out[-3000:]
+
+
+TIMER_LINES = ["Reorder_trans", "Reorder_out", "Reorder_in", "Trans_exec", "Packsend", "Packsend_trans",
+               "Unpackrecv", "Alltoall"]
+
+
+def timer_report(out):
+    """{phase: (avg, min, max)} from the TIMERS report lines."""
+    rep = {}
+    for line in out.splitlines():
+        m = re.match(r"^(\w+)\s*\(([-\d.e+]+) ([-\d.e+]+) ([-\d.e+]+)\)", line.strip())
+        if m:
+            rep[m.group(1)] = tuple(float(m.group(i)) for i in (2, 3, 4))
+    return rep
+
+
+@pytest.mark.parametrize("prog", ["cxx_timers_test3D_c2c", "cxx_timers_test3D_c2c_inplace"])
+def test_sample_timers_report(prog):
+    """The samples built with -DTIMERS call timers.init() before their timing
+    loop and timers.print(MPI_COMM_WORLD) after it (sample/C++/test3D_c2c.C:
+    223-224, 263-264): the report lists the reference's phases, the GPU time
+    of the fused reorder + transform passes lands in Reorder_trans /
+    Trans_exec, and the exchange phases of a 1x1 grid stay 0."""
+    r, out = run_sample(prog, {"stdin": "64 64 64 1 3\n"})
+    assert r.returncode == 0, out[-3000:]
+    assert "Results are correct" in out, out[-3000:]
+    assert "TIMERS (avg/min/max)" in out, out[-3000:]
+    rep = timer_report(out)
+    for k in TIMER_LINES:
+        assert k in rep, (k, out[-2000:])
+    busy = rep["Reorder_trans"][0] + rep["Trans_exec"][0]
+    assert busy > 0.0, out[-2000:]
+    # 3 repetitions of a forward and a backward 64^3 C2C: well under a second
+    assert busy < 1.0
+    for k in ("Packsend", "Packsend_trans", "Unpackrecv", "Alltoall"):
+        assert rep[k] == (0.0, 0.0, 0.0)
+
+
+def test_bench_c2c_timers():
+    """sample/C++/bench_c2c.C built with -DTIMERS: its proc-grid sweep on one
+    rank, the transform and the TIMERS report."""
+    r, out = run_sample("cxx_timers_bench_c2c", {"stdin": "64 64 64 1 2\n"})
+    assert r.returncode == 0, out[-3000:]
+    assert "incorrect" not in out, out[-3000:]
+    assert "TIMERS (avg/min/max)" in out, out[-3000:]
+    assert timer_report(out)["Reorder_trans"][0] + timer_report(out)["Trans_exec"][0] > 0.0
