@@ -71,15 +71,20 @@ int p3dfft_hip_batch_destroy(int bplan);
  * in memory order (0,1,2), x stride-1): u = inverse FFT of -f_k / |k|^2, with
  * the k = 0 mode set to 0.  prec: 4 (FP32) or 8 (FP64).
  *
- * The service runs its own thread and stream.  Requests are admitted in
- * arrival order when a slot (max_inflight) and their working memory (from a
- * device pool of pool_bytes allocated at creation) are free; every admitted
- * request advances one transform pass per step, and one step is one launch
- * over the current pass of all requests in flight, so requests that arrive
- * while others run join the next step.  rhs and sol are device arrays that
- * the caller keeps alive until the request is polled as done; rhs must be
- * complete when submitted.  A request's result does not depend on the other
- * requests in flight.  Times are seconds since the service was created. */
+ * The service runs its own thread and several streams ("lanes", 4 unless
+ * the environment variable P3DFFT_HIP_SERVE_LANES gives 1..8).  Requests
+ * are admitted in arrival order when a slot (max_inflight) and their
+ * working memory (from a device pool of pool_bytes allocated at creation)
+ * are free.  An admitted request is launched at once on the least busy
+ * lane: a grid of 2^18 points or more as its six pass launches, smaller
+ * grids waiting together as one group of six step-kernel launches over all
+ * their tiles.  Lanes run concurrently, so requests that arrive while others
+ * run start as soon as a lane has room and their work overlaps the others'
+ * on the device.  rhs and sol are device arrays that the caller keeps alive
+ * until the request is polled as done; rhs must be complete when submitted
+ * (e.g. the caller synchronized the stream that wrote it).  A request's
+ * result does not depend on the other requests in flight.  Times are
+ * seconds since the service was created. */
 int p3dfft_hip_poisson_server_create(int prec, long long pool_bytes, int max_inflight);
 /* Plan a grid size ahead of its first request (plans and tables stay
  * resident per size). */
@@ -91,9 +96,13 @@ long long p3dfft_hip_poisson_submit(int server, const int dims[3], const void *r
  * and returns their number; waits up to wait_ms for one if none is ready. */
 int p3dfft_hip_poisson_poll(int server, long long *ids, double *t_submit, double *t_admit, double *t_done,
                             int *ok, int cap, int wait_ms);
-/* Steps launched and tiles processed so far; returns 1 if the service has
- * failed, 0 if not, -1 for a bad handle. */
-int p3dfft_hip_poisson_server_stats(int server, long long *steps, long long *tiles);
+/* Kernel launches made and tiles processed so far; returns 1 if the
+ * service has failed, 0 if not, -1 for a bad handle. */
+int p3dfft_hip_poisson_server_stats(int server, long long *launches, long long *tiles);
+/* Requests waiting for admission, requests admitted and not yet done,
+ * launch groups in flight and lanes (for diagnostics).  Returns 0, or -1
+ * for a bad handle. */
+int p3dfft_hip_poisson_server_state(int server, int *pending, int *holding, int *groups_busy, int *lanes);
 /* Waits for every submitted request, then frees the service. */
 int p3dfft_hip_poisson_server_destroy(int server);
 /* One solve on the regular per-transform path (same plan math, one request

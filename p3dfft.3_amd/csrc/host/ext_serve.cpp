@@ -46,13 +46,20 @@ int p3dfft_hip_poisson_poll(int h, long long *ids, double *t_submit, double *t_a
   return p3e::poisson_poll(s, ids, t_submit, t_admit, t_done, ok, cap, wait_ms);
 }
 
-int p3dfft_hip_poisson_server_stats(int h, long long *steps, long long *tiles) {
+int p3dfft_hip_poisson_server_stats(int h, long long *launches, long long *tiles) {
   p3e::PoissonServer *s = server_of(h);
   if (!s) return -1;
   std::lock_guard<std::mutex> g(s->mu);
-  if (steps) *steps = s->nsteps;
+  if (launches) *launches = s->nlaunch;
   if (tiles) *tiles = s->ntiles;
   return s->failed ? 1 : 0;
+}
+
+int p3dfft_hip_poisson_server_state(int h, int *pending, int *holding, int *groups_busy, int *lanes) {
+  p3e::PoissonServer *s = server_of(h);
+  if (!s) return -1;
+  p3e::poisson_state(s, pending, holding, groups_busy, lanes);
+  return 0;
 }
 
 int p3dfft_hip_poisson_server_destroy(int h) {

@@ -1,6 +1,7 @@
 // Poisson service: see serve.h.
 #include "serve.h"
 
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -85,53 +86,161 @@ static void pool_free(PoissonServer *s, size_t off, size_t n) {
   }
 }
 
-// ------------------------------------------------------------------ steps
-static bool ensure_cap(PoissonServer::Step &b, size_t ntab, size_t nwork) {
-  if (ntab > b.cap_tab) {
-    if (b.h_tab) (void)hipHostFree(b.h_tab);
-    if (b.d_tab) (void)hipFree(b.d_tab);
-    b.h_tab = nullptr;
-    b.d_tab = nullptr;
-    const size_t c = std::max(ntab, 2 * b.cap_tab);
-    if (hipHostMalloc((void **)&b.h_tab, c * sizeof(PassArgs), hipHostMallocDefault) != hipSuccess ||
-        hipMalloc((void **)&b.d_tab, c * sizeof(PassArgs)) != hipSuccess) {
-      b.cap_tab = 0;
+// ------------------------------------------------------------------ launch checks
+bool check_launch(const PassArgs *tab, const PassDesc *const *descs, size_t ntab, const WorkItem *work,
+                  size_t nwork) {
+  for (size_t i = 0; i < ntab; i++) {
+    const PassArgs &a = tab[i];
+    // every tile loop of the kernels strides by nthreads
+    if (a.nthreads < 1 || a.B < 1 || a.nrad > P3E_MAX_RADIX_PASSES) {
+      set_error("Poisson service: pass %zu has nthreads %d, B %d, %d radix passes", i, a.nthreads, a.B, a.nrad);
       return false;
     }
-    b.cap_tab = c;
+    if (descs && !tables_ready(*descs[i])) {
+      set_error("Poisson service: pass %zu has a table without a device copy", i);
+      return false;
+    }
   }
-  if (nwork > b.cap_work) {
-    if (b.h_work) (void)hipHostFree(b.h_work);
-    if (b.d_work) (void)hipFree(b.d_work);
-    b.h_work = nullptr;
-    b.d_work = nullptr;
-    const size_t c = std::max(nwork, 2 * b.cap_work);
-    if (hipHostMalloc((void **)&b.h_work, c * sizeof(WorkItem), hipHostMallocDefault) != hipSuccess ||
-        hipMalloc((void **)&b.d_work, c * sizeof(WorkItem)) != hipSuccess) {
-      b.cap_work = 0;
+  for (size_t w = 0; w < nwork; w++) {
+    if (work[w].pass >= ntab) {
+      set_error("Poisson service: work item %zu names pass %u of %zu", w, work[w].pass, ntab);
       return false;
     }
-    b.cap_work = c;
+    if (descs && work[w].tile >= descs[work[w].pass]->nblocks) {
+      set_error("Poisson service: work item %zu names tile %u of %u", w, work[w].tile,
+                descs[work[w].pass]->nblocks);
+      return false;
+    }
   }
   return true;
 }
 
-// Called with s->mu held: the requests of a completed step are done.
-static void retire(PoissonServer *s, PoissonServer::Step &b) {
+// ------------------------------------------------------------------ groups
+static bool grow(void **h, void **d, size_t *cap, size_t need, size_t esz) {
+  if (need <= *cap) return true;
+  // the group is not in flight: its previous copies have completed
+  if (*h) (void)hipHostFree(*h);
+  if (*d) (void)hipFree(*d);
+  *h = *d = nullptr;
+  const size_t c = std::max(need, 2 * *cap);
+  if (hipHostMalloc(h, c * esz, hipHostMallocDefault) != hipSuccess || hipMalloc(d, c * esz) != hipSuccess) {
+    *cap = 0;
+    return false;
+  }
+  *cap = c;
+  return true;
+}
+
+// Working buffers of pass ph of request r: the rhs, the two halves of its
+// pool block in turn, and the solution.
+static void bind(const PoissonServer *s, const PoissonRequest *r, int ph, PassArgs &a) {
+  const PoissonPlan *pl = r->plan;
+  const int last = (int)pl->passes.size() - 1;
+  char *w0 = s->pool + r->off, *w1 = w0 + pl->work_bytes;
+  a.in = ph == 0 ? r->rhs : ((ph - 1) % 2 ? w1 : w0);
+  a.out = ph == last ? r->sol : (ph % 2 ? w1 : w0);
+  a.deriv = 0;
+}
+
+// Launches a group on its lane (s->mu not held; the group is reserved for
+// this thread): a large request's passes as their own launches, or the
+// passes of a batch of small requests as one step-kernel launch per pass
+// index; then the group's event.
+static int launch_group(PoissonServer *s, PoissonGroup &g) {
+  const int prec = s->prec;
+  hipStream_t st = s->lanes[g.lane];
+  if (g.reqs.size() == 1 && g.reqs[0]->plan->dedicated) {
+    const PoissonRequest *r = g.reqs[0];
+    const PoissonPlan *pl = r->plan;
+    for (int ph = 0; ph < (int)pl->passes.size(); ph++) {
+      const PassDesc &pd = pl->passes[ph];
+      PassArgs a = pd.args;
+      bind(s, r, ph, a);
+      const PassDesc *dp = &pd;
+      if (!check_launch(&a, &dp, 1, nullptr, 0)) return -1;
+      if (p3e_launch_pass(&a, prec, pd.lds_bytes, pd.threads, pd.nblocks, st)) {
+        set_error("Poisson service: pass launch failed");
+        return -1;
+      }
+      s->nlaunch++;
+      s->ntiles += pd.nblocks;
+    }
+  } else {
+    // passes of every request, pass index major: entry ph * n + i
+    const size_t n = g.reqs.size();
+    const int npass = (int)g.reqs[0]->plan->passes.size();
+    size_t nwork = 0;
+    for (const PoissonRequest *r : g.reqs) {
+      if ((int)r->plan->passes.size() != npass) {
+        set_error("Poisson service: requests of one group differ in their number of passes");
+        return -1;
+      }
+      for (const PassDesc &pd : r->plan->passes) nwork += pd.nblocks;
+    }
+    const size_t ntab = n * npass;
+    if (!grow((void **)&g.h_tab, (void **)&g.d_tab, &g.cap_tab, ntab, sizeof(PassArgs)) ||
+        !grow((void **)&g.h_work, (void **)&g.d_work, &g.cap_work, nwork, sizeof(WorkItem))) {
+      set_error("Poisson service: allocation of launch tables failed");
+      return -1;
+    }
+    std::vector<const PassDesc *> descs(ntab);
+    std::vector<size_t> w_off(npass + 1, 0);
+    std::vector<size_t> lds(npass, 0);
+    size_t w = 0;
+    for (int ph = 0; ph < npass; ph++) {
+      w_off[ph] = w;
+      for (size_t i = 0; i < n; i++) {
+        const PassDesc &pd = g.reqs[i]->plan->passes[ph];
+        const size_t t = ph * n + i;
+        PassArgs a = pd.args;
+        bind(s, g.reqs[i], ph, a);
+        g.h_tab[t] = a;
+        descs[t] = &pd;
+        lds[ph] = std::max(lds[ph], pd.lds_bytes);
+        for (unsigned k = 0; k < pd.nblocks; k++) g.h_work[w++] = WorkItem{(uint32_t)t, k};
+      }
+    }
+    w_off[npass] = w;
+    if (!check_launch(g.h_tab, descs.data(), ntab, g.h_work, nwork)) return -1;
+    if (hipMemcpyAsync(g.d_tab, g.h_tab, ntab * sizeof(PassArgs), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(g.d_work, g.h_work, nwork * sizeof(WorkItem), hipMemcpyHostToDevice, st) != hipSuccess) {
+      set_error("Poisson service: table upload failed");
+      return -1;
+    }
+    for (int ph = 0; ph < npass; ph++) {
+      const int nw = (int)(w_off[ph + 1] - w_off[ph]);
+      const unsigned grid = persistent_grid(prec, lds[ph], nw);
+      if (p3e_launch_step(g.d_tab, g.d_work + w_off[ph], nw, prec, lds[ph], grid, st)) {
+        set_error("Poisson service: step launch failed");
+        return -1;
+      }
+      s->nlaunch++;
+      s->ntiles += nw;
+    }
+  }
+  if (hipEventRecord(g.done, st) != hipSuccess) {
+    set_error("Poisson service: event record failed");
+    return -1;
+  }
+  s->ngroups++;
+  return 0;
+}
+
+// Called with s->mu held: the requests of a completed group are done.
+static void retire(PoissonServer *s, PoissonGroup &g) {
   const double t = now_s() - s->t0;
-  for (PoissonRequest *r : b.finishing) {
+  for (PoissonRequest *r : g.reqs) {
     r->t_done = t;
     pool_free(s, r->off, 2 * r->plan->work_bytes);
     s->holding--;
-    if (r->plan->dedicated) s->holding_large--;
     s->completed.push_back(r);
   }
-  b.finishing.clear();
-  b.busy = false;
+  g.reqs.clear();
+  g.busy = false;
   s->done_cv.notify_all();
 }
 
-static void fail_all(PoissonServer *s, const char *msg) {
+static void fail_all(PoissonServer *s, const std::string &msg) {
   s->failed = true;
   s->error = msg;
   const double t = now_s() - s->t0;
@@ -140,134 +249,94 @@ static void fail_all(PoissonServer *s, const char *msg) {
     r->t_done = t;
     s->completed.push_back(r);
   };
-  for (PoissonRequest *r : s->active) finish(r);
   for (PoissonRequest *r : s->pending) finish(r);
-  for (auto &b : s->steps) {
-    for (PoissonRequest *r : b.finishing) finish(r);
-    b.finishing.clear();
-  }
-  s->active.clear();
   s->pending.clear();
+  // groups in flight keep their requests until the device is done with
+  // them (the service thread drains them before it exits)
   s->done_cv.notify_all();
 }
 
-// Builds and launches one step over the current pass of every active
-// request (s->mu not held; only the service thread touches `active`): the
-// passes of large grids as their own launches, then one step-kernel launch
-// over the tiles of all the others.
-static int launch_step(PoissonServer *s, PoissonServer::Step &b) {
-  const int prec = s->prec;
-  size_t ntab = 0, nwork = 0, lds = 0;
-  for (PoissonRequest *r : s->active) {
-    if (r->plan->dedicated) continue;
-    const PassDesc &pd = r->plan->passes[r->phase];
-    ntab++;
-    nwork += pd.nblocks;
-    lds = std::max(lds, pd.lds_bytes);
+// A free group on the lane with the fewest groups in flight, or null.
+static PoissonGroup *free_group(PoissonServer *s) {
+  std::vector<int> busy(s->nlanes, 0);
+  for (const PoissonGroup &g : s->groups)
+    if (g.busy) busy[g.lane]++;
+  PoissonGroup *best = nullptr;
+  for (PoissonGroup &g : s->groups)
+    if (!g.busy && (!best || busy[g.lane] < busy[best->lane])) best = &g;
+  return best;
+}
+
+// Called with s->mu held: takes the next requests that can start into g
+// (one large request, or consecutive small ones up to SMALL_GROUP_MAX), with
+// their working memory; false if the request at the head cannot start yet.
+static bool admit(PoissonServer *s, PoissonGroup &g) {
+  const double t = now_s() - s->t0;
+  while (!s->pending.empty() && s->holding < s->max_inflight && (int)g.reqs.size() < SMALL_GROUP_MAX) {
+    PoissonRequest *r = s->pending.front();
+    if (!g.reqs.empty() && (r->plan->dedicated || g.reqs[0]->plan->dedicated)) break;
+    if (!g.reqs.empty() && r->plan->passes.size() != g.reqs[0]->plan->passes.size()) break;
+    size_t off;
+    if (!pool_alloc(s, 2 * r->plan->work_bytes, &off)) break;
+    r->off = off;
+    r->t_admit = t;
+    s->pending.pop_front();
+    s->holding++;
+    g.reqs.push_back(r);
+    if (r->plan->dedicated) break;
   }
-  if (!ensure_cap(b, std::max<size_t>(ntab, 1), std::max<size_t>(nwork, 1))) return -1;
-  size_t w = 0, t_i = 0;
-  for (PoissonRequest *r : s->active) {
-    const PoissonPlan *pl = r->plan;
-    const int ph = r->phase, last = (int)pl->passes.size() - 1;
-    char *w0 = s->pool + r->off, *w1 = w0 + pl->work_bytes;
-    const PassDesc &pd = pl->passes[ph];
-    PassArgs a = pd.args;
-    a.in = ph == 0 ? r->rhs : ((ph - 1) % 2 ? w1 : w0);
-    a.out = ph == last ? r->sol : (ph % 2 ? w1 : w0);
-    a.deriv = 0;
-    if (pl->dedicated) {
-      if (p3e_launch_pass(&a, prec, pd.lds_bytes, pd.threads, pd.nblocks, s->st)) return -1;
-      s->ntiles += pd.nblocks;
-      continue;
-    }
-    b.h_tab[t_i] = a;
-    for (unsigned t = 0; t < pd.nblocks; t++) b.h_work[w++] = WorkItem{(uint32_t)t_i, t};
-    t_i++;
-  }
-  if (nwork) {
-    if (hipMemcpyAsync(b.d_tab, b.h_tab, ntab * sizeof(PassArgs), hipMemcpyHostToDevice, s->st) != hipSuccess ||
-        hipMemcpyAsync(b.d_work, b.h_work, nwork * sizeof(WorkItem), hipMemcpyHostToDevice, s->st) != hipSuccess)
-      return -1;
-    const unsigned grid = persistent_grid(prec, lds, (int)nwork);
-    if (p3e_launch_step(b.d_tab, b.d_work, (int)nwork, prec, lds, grid, s->st)) return -1;
-  }
-  if (hipEventRecord(b.done, s->st) != hipSuccess) return -1;
-  s->nsteps++;
-  s->ntiles += (long long)nwork;
-  return 0;
+  return !g.reqs.empty();
 }
 
 static void service_loop(PoissonServer *s) {
   (void)hipSetDevice(s->device);
   std::unique_lock<std::mutex> lk(s->mu);
-  while (!s->failed) {
-    // completed steps
-    for (int k = 0; k < SERVE_DEPTH; k++) {
-      PoissonServer::Step &b = s->steps[(s->cur + k) % SERVE_DEPTH];
-      if (b.busy && hipEventQuery(b.done) == hipSuccess) retire(s, b);
-    }
-    // admission: arrival order, a slot, working memory, and for a large
-    // grid one of LARGE_INFLIGHT places
-    while (!s->pending.empty() && s->holding < s->max_inflight) {
-      PoissonRequest *r = s->pending.front();
-      if (r->plan->dedicated && s->holding_large >= LARGE_INFLIGHT) break;
-      size_t off;
-      if (!pool_alloc(s, 2 * r->plan->work_bytes, &off)) break;
-      r->off = off;
-      r->phase = 0;
-      r->t_admit = now_s() - s->t0;
-      s->pending.pop_front();
-      s->active.push_back(r);
-      s->holding++;
-      if (r->plan->dedicated) s->holding_large++;
-    }
-    if (s->active.empty()) {
-      PoissonServer::Step *oldest = nullptr;
-      for (int k = 0; k < SERVE_DEPTH; k++)
-        if (s->steps[(s->cur + k) % SERVE_DEPTH].busy) {
-          oldest = &s->steps[(s->cur + k) % SERVE_DEPTH];
-          break;
-        }
-      if (oldest) {
-        lk.unlock();
-        const hipError_t e = hipEventSynchronize(oldest->done);
-        lk.lock();
-        if (e != hipSuccess) fail_all(s, "Poisson service: a step failed on the device");
-        else retire(s, *oldest);
-        continue;
+  for (;;) {
+    bool any_busy = false;
+    for (PoissonGroup &g : s->groups) {
+      if (!g.busy) continue;
+      const hipError_t e = hipEventQuery(g.done);
+      if (e == hipSuccess) {
+        retire(s, g);
+      } else if (e != hipErrorNotReady) {
+        // the device reported an error: every request of the group fails
+        for (PoissonRequest *r : g.reqs) r->failed = true;
+        retire(s, g);
+        if (!s->failed) fail_all(s, "Poisson service: a launch group failed on the device");
+      } else {
+        any_busy = true;
       }
-      if (s->stop && s->pending.empty()) break;
-      s->cv.wait_for(lk, std::chrono::milliseconds(50));
-      continue;
     }
-    PoissonServer::Step &b = s->steps[s->cur];
-    if (b.busy) {
+    while (!s->failed && !s->pending.empty()) {
+      PoissonGroup *g = free_group(s);
+      if (!g || !admit(s, *g)) break;
+      g->busy = true;  // reserved: only this thread launches and retires groups
       lk.unlock();
-      const hipError_t e = hipEventSynchronize(b.done);
+      const int rc = launch_group(s, *g);
       lk.lock();
-      if (e != hipSuccess) {
-        fail_all(s, "Poisson service: a step failed on the device");
+      if (rc) {
+        // nothing of a failed group reached the device past its failing
+        // launch; its requests fail with the rest
+        const std::string msg = last_error();
+        s->ngroups++;
+        for (PoissonRequest *r : g->reqs) r->failed = true;
+        (void)hipStreamSynchronize(s->lanes[g->lane]);
+        retire(s, *g);
+        fail_all(s, msg);
         break;
       }
-      retire(s, b);
+      any_busy = true;
     }
-    lk.unlock();
-    const int rc = launch_step(s, b);
-    lk.lock();
-    if (rc) {
-      fail_all(s, "Poisson service: step launch failed");
-      break;
+    if (!any_busy && s->pending.empty() && (s->stop || s->failed)) break;
+    if (any_busy) {
+      // groups in flight: poll their events (a submission is seen at the
+      // next turn of the loop)
+      lk.unlock();
+      std::this_thread::yield();
+      lk.lock();
+    } else {
+      s->cv.wait_for(lk, std::chrono::milliseconds(50));
     }
-    b.busy = true;
-    std::vector<PoissonRequest *> still;
-    for (PoissonRequest *r : s->active) {
-      r->phase++;
-      if (r->phase == (int)r->plan->passes.size()) b.finishing.push_back(r);
-      else still.push_back(r);
-    }
-    s->active.swap(still);
-    s->cur = (s->cur + 1) % SERVE_DEPTH;
   }
 }
 
@@ -279,27 +348,33 @@ PoissonServer *poisson_server_create(int prec, size_t pool_bytes, int max_inflig
   PoissonServer *s = new PoissonServer();
   s->prec = prec;
   s->max_inflight = max_inflight;
+  s->nlanes = SERVE_LANES;
+  if (const char *e = getenv("P3DFFT_HIP_SERVE_LANES")) {
+    const int v = atoi(e);
+    if (v >= 1 && v <= 8) s->nlanes = v;
+  }
   s->pool = nullptr;
   s->pool_bytes = pool_bytes & ~(size_t)255;
-  s->st = nullptr;
-  s->cur = 0;
   s->holding = 0;
-  s->holding_large = 0;
   s->next_id = 0;
-  s->nsteps = s->ntiles = 0;
+  s->ngroups = s->nlaunch = s->ntiles = 0;
   s->stop = s->failed = false;
   s->t0 = now_s();
-  for (auto &b : s->steps) {
-    b.h_tab = b.d_tab = nullptr;
-    b.h_work = b.d_work = nullptr;
-    b.cap_tab = b.cap_work = 0;
-    b.busy = false;
-    b.done = nullptr;
+  s->lanes.assign(s->nlanes, nullptr);
+  s->groups.resize((size_t)s->nlanes * LANE_DEPTH);
+  for (size_t i = 0; i < s->groups.size(); i++) {
+    PoissonGroup &g = s->groups[i];
+    g.lane = (int)(i % s->nlanes);
+    g.busy = false;
+    g.done = nullptr;
+    g.h_tab = g.d_tab = nullptr;
+    g.h_work = g.d_work = nullptr;
+    g.cap_tab = g.cap_work = 0;
   }
-  bool ok = hipGetDevice(&s->device) == hipSuccess &&
-            hipMalloc((void **)&s->pool, s->pool_bytes) == hipSuccess &&
-            hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) == hipSuccess;
-  for (auto &b : s->steps) ok = ok && hipEventCreateWithFlags(&b.done, hipEventDisableTiming) == hipSuccess;
+  bool ok = hipGetDevice(&s->device) == hipSuccess && hipMalloc((void **)&s->pool, s->pool_bytes) == hipSuccess;
+  for (hipStream_t &st : s->lanes) ok = ok && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess;
+  for (PoissonGroup &g : s->groups)
+    ok = ok && hipEventCreateWithFlags(&g.done, hipEventDisableTiming) == hipSuccess;
   if (!ok) {
     set_error("Poisson service: device allocation of %zu bytes failed", s->pool_bytes);
     poisson_server_destroy(s);
@@ -349,7 +424,6 @@ long long poisson_submit(PoissonServer *s, const int dims[3], const void *rhs, v
   r->rhs = rhs;
   r->sol = sol;
   r->off = 0;
-  r->phase = 0;
   r->t_submit = now_s() - s->t0;
   r->t_admit = r->t_done = -1;
   r->failed = false;
@@ -378,6 +452,16 @@ int poisson_poll(PoissonServer *s, long long *ids, double *t_submit, double *t_a
   return n;
 }
 
+void poisson_state(PoissonServer *s, int *pending, int *holding, int *groups_busy, int *lanes) {
+  std::lock_guard<std::mutex> g(s->mu);
+  int busy = 0;
+  for (const PoissonGroup &gr : s->groups) busy += gr.busy ? 1 : 0;
+  if (pending) *pending = (int)s->pending.size();
+  if (holding) *holding = s->holding;
+  if (groups_busy) *groups_busy = busy;
+  if (lanes) *lanes = s->nlanes;
+}
+
 void poisson_server_destroy(PoissonServer *s) {
   if (!s) return;
   {
@@ -386,21 +470,22 @@ void poisson_server_destroy(PoissonServer *s) {
     s->cv.notify_all();
   }
   if (s->th.joinable()) s->th.join();
-  if (s->st) (void)hipStreamSynchronize(s->st);
+  for (hipStream_t st : s->lanes)
+    if (st) (void)hipStreamSynchronize(st);
   for (PoissonRequest *r : s->completed) delete r;
   for (PoissonRequest *r : s->pending) delete r;
-  for (PoissonRequest *r : s->active) delete r;
-  for (auto &b : s->steps) {
-    for (PoissonRequest *r : b.finishing) delete r;
-    if (b.h_tab) (void)hipHostFree(b.h_tab);
-    if (b.d_tab) (void)hipFree(b.d_tab);
-    if (b.h_work) (void)hipHostFree(b.h_work);
-    if (b.d_work) (void)hipFree(b.d_work);
-    if (b.done) (void)hipEventDestroy(b.done);
+  for (PoissonGroup &g : s->groups) {
+    for (PoissonRequest *r : g.reqs) delete r;
+    if (g.h_tab) (void)hipHostFree(g.h_tab);
+    if (g.d_tab) (void)hipFree(g.d_tab);
+    if (g.h_work) (void)hipHostFree(g.h_work);
+    if (g.d_work) (void)hipFree(g.d_work);
+    if (g.done) (void)hipEventDestroy(g.done);
   }
   for (auto &kv : s->plans) free_plan(kv.second);
   if (s->pool) (void)hipFree(s->pool);
-  if (s->st) (void)hipStreamDestroy(s->st);
+  for (hipStream_t st : s->lanes)
+    if (st) (void)hipStreamDestroy(st);
   delete s;
 }
 

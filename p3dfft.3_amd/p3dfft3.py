@@ -314,7 +314,19 @@ def poisson_server_stats(h):
                                                   ctypes.POINTER(ctypes.c_longlong)]
     a, b = ctypes.c_longlong(), ctypes.c_longlong()
     rc = L.p3dfft_hip_poisson_server_stats(h, ctypes.byref(a), ctypes.byref(b))
-    return {"steps": a.value, "tiles": b.value, "failed": rc == 1}
+    return {"launches": a.value, "tiles": b.value, "failed": rc == 1}
+
+
+def poisson_server_state(h):
+    """{pending, holding, groups_busy, lanes} of a service (diagnostics)."""
+    L = lib()
+    IP = ctypes.POINTER(ctypes.c_int)
+    L.p3dfft_hip_poisson_server_state.argtypes = [ctypes.c_int, IP, IP, IP, IP]
+    v = [ctypes.c_int() for _ in range(4)]
+    if L.p3dfft_hip_poisson_server_state(h, *[ctypes.byref(x) for x in v]) != 0:
+        check()
+        raise RuntimeError("p3dfft_hip_poisson_server_state failed")
+    return dict(zip(("pending", "holding", "groups_busy", "lanes"), (x.value for x in v)))
 
 
 def poisson_server_destroy(h):

@@ -1,14 +1,17 @@
 // CPU test of the Poisson service's scheduling (test infrastructure): the
-// product's serve.cpp and planner, the mock HIP runtime, and kernels that
-// record which buffers each pass read and wrote.  Checks, for several
-// in-flight limits and pool sizes: every request completes (no hang); each
-// request's passes ran in order and form the chain rhs -> ... -> sol; no
-// pass of another request wrote a buffer between its writer and its reader
-// in a chain; two requests never held overlapping working memory.
+// product's serve.cpp and planner, the mock HIP runtime (one concurrent
+// queue per stream; frees and pinned allocations synchronize the device),
+// and kernels that record which buffers each pass read and wrote.  Checks,
+// for several lane counts, in-flight limits, pool sizes and device speeds:
+// every request completes (no hang); each request's passes ran in order and
+// form the chain rhs -> ... -> sol; no pass wrote a buffer between its
+// writer and its reader in a chain; no more than max_inflight requests held
+// working memory at once.
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
 #include <functional>
 #include <map>
@@ -23,7 +26,7 @@
 
 using namespace p3e;
 
-void mock_enqueue(std::function<void()> f);
+void mock_enqueue(hipStream_t st, std::function<void()> f);
 void mock_drain();
 void mock_set_delay(int us);
 void mock_shutdown();
@@ -41,16 +44,16 @@ static void record(const void *in, void *out) {
   g_execs.push_back(Exec{in, out});
 }
 
-extern "C" int p3e_launch_pass(const PassArgs *a, int, size_t, int, unsigned int, hipStream_t) {
+extern "C" int p3e_launch_pass(const PassArgs *a, int, size_t, int, unsigned int, hipStream_t st) {
   const void *in = a->in;
   void *out = a->out;
-  mock_enqueue([in, out] { record(in, out); });
+  mock_enqueue(st, [in, out] { record(in, out); });
   return 0;
 }
 
 extern "C" int p3e_launch_step(const void *table, const void *work, int nwork, int, size_t, unsigned int,
-                               hipStream_t) {
-  mock_enqueue([table, work, nwork] {
+                               hipStream_t st) {
+  mock_enqueue(st, [table, work, nwork] {
     const PassArgs *t = (const PassArgs *)table;
     const WorkItem *w = (const WorkItem *)work;
     std::set<uint32_t> seen;
@@ -81,13 +84,16 @@ hipStream_t default_stream() { return nullptr; }
 }  // namespace p3e
 
 // ---------------------------------------------------------------- scenario
-static int scenario(int max_inflight, size_t pool, int nreq, unsigned seed) {
+static int scenario(int lanes, int max_inflight, size_t pool, int nreq, unsigned seed) {
   {
     std::lock_guard<std::mutex> g(g_rec_mu);
     g_execs.clear();
   }
   std::mt19937 rng(seed);
   const int edges[] = {16, 24, 32, 40, 48, 64, 96, 128};
+  char lv[8];
+  snprintf(lv, sizeof(lv), "%d", lanes);
+  setenv("P3DFFT_HIP_SERVE_LANES", lv, 1);
   PoissonServer *s = poisson_server_create(4, pool, max_inflight);
   if (!s) {
     printf("create failed: %s\n", last_error());
@@ -114,11 +120,9 @@ static int scenario(int max_inflight, size_t pool, int nreq, unsigned seed) {
   std::thread dog([&] {
     for (int t = 0; t < 100 && !finished; t++) std::this_thread::sleep_for(std::chrono::milliseconds(100));
     if (finished) return;
-    std::unique_lock<std::mutex> lk(s->mu, std::try_to_lock);
-    printf("stall: lock %s pending %zu active %zu holding %d completed %zu cur %d busy", lk.owns_lock() ? "free" : "held",
-           s->pending.size(), s->active.size(), s->holding, s->completed.size(), s->cur);
-    for (auto &b : s->steps) printf(" %d/%zu", (int)b.busy, b.finishing.size());
-    printf("\n");
+    int pend, hold, busy, nl;
+    poisson_state(s, &pend, &hold, &busy, &nl);
+    printf("stall: pending %d holding %d groups in flight %d lanes %d\n", pend, hold, busy, nl);
     fflush(stdout);
   });
   // submit in bursts, polling between, as a client would
@@ -158,20 +162,18 @@ static int scenario(int max_inflight, size_t pool, int nreq, unsigned seed) {
   dog.join();
   poisson_server_destroy(s);
   mock_drain();
-  // large grids (dedicated launches): at most LARGE_INFLIGHT held at once
+  // admitted requests held working memory at most max_inflight at once
   {
     std::vector<std::pair<double, int>> ev;  // (time, +1 admit / -1 done), done first at equal times
     for (int i = 0; i < nreq; i++) {
-      const long long pts = (long long)reqs[i].dims[0] * reqs[i].dims[1] * reqs[i].dims[2];
-      if (pts < DEDICATED_POINTS) continue;
       ev.push_back({span[ids[i]].first, +1});
       ev.push_back({span[ids[i]].second, -1});
     }
     std::sort(ev.begin(), ev.end());
     int cur = 0, peak = 0;
     for (auto &e : ev) peak = std::max(peak, cur += e.second);
-    if (peak > LARGE_INFLIGHT) {
-      printf("%d large requests held at once (limit %d)\n", peak, LARGE_INFLIGHT);
+    if (peak > max_inflight) {
+      printf("%d requests held at once (limit %d)\n", peak, max_inflight);
       return 6;
     }
   }
@@ -207,7 +209,8 @@ static int scenario(int max_inflight, size_t pool, int nreq, unsigned seed) {
       }
     }
   }
-  printf("ok: max_inflight %d pool %zu requests %d passes %zu\n", max_inflight, pool, nreq, ex.size());
+  printf("ok: lanes %d max_inflight %d pool %zu requests %d passes %zu\n", lanes, max_inflight, pool, nreq,
+         ex.size());
   return 0;
 }
 
@@ -215,17 +218,18 @@ int main() {
   setvbuf(stdout, NULL, _IONBF, 0);
   int rc = 0;
   const size_t big = 1ull << 30, tight = 40ull << 20;  // tight: one 2^20-point request plus a few small ones
-  for (int delay : {30, 0, 2})
-    for (int mi : {1, 2, 4, 64})
-      for (size_t pool : {big, tight}) {
-        mock_set_delay(delay);
-        printf("device delay %d us: ", delay);
-        rc = scenario(mi, pool, 40, 1234u + mi + delay);
-        if (rc) {
-          mock_shutdown();
-          return rc;
+  for (int lanes : {1, 4})
+    for (int delay : {30, 0, 2})
+      for (int mi : {1, 2, 4, 64})
+        for (size_t pool : {big, tight}) {
+          mock_set_delay(delay);
+          printf("device delay %d us: ", delay);
+          rc = scenario(lanes, mi, pool, 40, 1234u + mi + delay + 7 * lanes);
+          if (rc) {
+            mock_shutdown();
+            return rc;
+          }
         }
-      }
   mock_shutdown();
   return 0;
 }
