@@ -38,22 +38,25 @@ def b_alg(n, e):
     return 2 * (vr + 5 * vc)
 
 
-def cpu_baseline(n_sample=256):
+def cpu_baseline(n=512):
     """The FP64 oracle (single-threaded C restatement of the reference's
-    FFTW path) on a bounded sample: one R2C+C2R round trip at n_sample^3 with
-    the same memory orders.  Rate = B_alg(n_sample, FP32) / t."""
+    FFTW path) on the headline grid: one R2C+C2R round trip at n^3 with the
+    same memory orders.  The oracle computes in FP64, so its rate counts FP64
+    bytes: B_alg(n, 8) / t."""
     import numpy as np
     import oracle_binding as orc
-    x = np.random.default_rng(2).uniform(-1.0, 1.0, (n_sample,) * 3)
+    x = np.random.default_rng(2).uniform(-1.0, 1.0, (n,) * 3)
     t0 = time.perf_counter()
-    spec = orc.exec3d([5, 9, 9], [n_sample] * 3, [2, 1, 0], [1, 2, 0], x)
-    back = orc.exec3d([7, 11, 11], [n_sample // 2 + 1, n_sample, n_sample], [1, 2, 0], [2, 1, 0], spec)
+    spec = orc.exec3d([5, 9, 9], [n] * 3, [2, 1, 0], [1, 2, 0], x)
+    back = orc.exec3d([7, 11, 11], [n // 2 + 1, n, n], [1, 2, 0], [2, 1, 0], spec)
     t = time.perf_counter() - t0
-    err = float(np.abs(back / n_sample ** 3 - x).max())
-    return {"value": round(b_alg(n_sample, 4) / t / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
-            "sample": f"oracle/ FP64 C restatement, one R2C+C2R round trip at {n_sample}^3 "
-                      f"(mem order (2,1,0)->(1,2,0)), single thread, {t:.1f} s, roundtrip max err {err:.1e}; "
-                      f"rate = B_alg({n_sample}^3, FP32) / t"}
+    del spec
+    err = float(np.abs(back / n ** 3 - x).max())
+    return {"value": round(b_alg(n, 8) / t / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+            "ms_per_roundtrip": round(t * 1e3, 1),
+            "sample": f"oracle/ FP64 C restatement of the reference's FFTW path (FFTW is not in the image), one "
+                      f"R2C+C2R round trip at {n}^3 (mem order (2,1,0)->(1,2,0)), single thread, {t:.1f} s, "
+                      f"round trip max err {err:.1e}; rate = B_alg({n}^3, FP64 bytes) / t"}
 
 
 def batch_bench(P, torch, reps=10):
@@ -182,6 +185,12 @@ def serving_bench(P, torch, rates=(50, 200, 800), seconds=(4.0, 2.0, 1.0), nshap
             out.append((rid, ts, td))
             keep.pop(rid, None)
 
+    def check_deadline(t_end, what):
+        # a stalled service ends the run with its state instead of silence
+        if time.perf_counter() > t_end:
+            raise RuntimeError(f"Poisson service stalled during {what}: {len(keep)} requests outstanding, "
+                               f"state {P.poisson_server_state(h)}, stats {P.poisson_server_stats(h)}")
+
     result = {"workload": "periodic Poisson solves, FP32, grid edges log-uniform in [64,384] (multiples of 8, "
                           "aspect <= 2:1), 32 shapes (seed 5), rhs 8 Gaussians minus mean",
               "rates": []}
@@ -191,8 +200,10 @@ def serving_bench(P, torch, rates=(50, 200, 800), seconds=(4.0, 2.0, 1.0), nshap
     for i, d in enumerate(shapes):
         o = torch.empty_like(rhs[i])
         keep[P.poisson_submit(h, d, rhs[i].data_ptr(), o.data_ptr())] = o
+    t_end = time.perf_counter() + 120
     while keep:
         drain(done, keep)
+        check_deadline(t_end, "warm-up")
         time.sleep(0.001)
     for rate, dur in zip(rates, seconds):
         n = max(1, int(rate * dur))
@@ -200,8 +211,10 @@ def serving_bench(P, torch, rates=(50, 200, 800), seconds=(4.0, 2.0, 1.0), nshap
         which = rng.integers(0, nshapes, n)
         keep, done, sub = {}, [], {}
         t0 = time.perf_counter()
+        t_end = t0 + dur + 60
         i = 0
         while i < n or keep:
+            check_deadline(t_end, f"the {rate} req/s run")
             now = time.perf_counter() - t0
             while i < n and t_arr[i] <= now:
                 k = int(which[i])
@@ -229,8 +242,10 @@ def serving_bench(P, torch, rates=(50, 200, 800), seconds=(4.0, 2.0, 1.0), nshap
     t0 = time.perf_counter()
     for k, o in zip(which, outs):
         keep[P.poisson_submit(h, shapes[int(k)], rhs[int(k)].data_ptr(), o.data_ptr())] = o
+    t_end = time.perf_counter() + 60
     while keep:
         drain(done, keep)
+        check_deadline(t_end, "the closed loop")
         time.sleep(0.0002)
     t_srv = time.perf_counter() - t0
     # accuracy of the service's results on a sample vs torch FP64
@@ -249,25 +264,62 @@ def serving_bench(P, torch, rates=(50, 200, 800), seconds=(4.0, 2.0, 1.0), nshap
         P.poisson_solve(shapes[int(k)], rhs[int(k)].data_ptr(), o.data_ptr(), 4, st, False)
     torch.cuda.synchronize()
     t_per = time.perf_counter() - t0
-    steps = P.poisson_server_stats(h)
+    stats = P.poisson_server_stats(h)
+    lanes = P.poisson_server_state(h)["lanes"]
     P.poisson_server_destroy(h)
     balg = sum(alg[int(k)] for k in which)
     result.update({"closed_loop_requests": closed, "service_solves_per_s": round(closed / t_srv, 1),
                    "per_request_path_solves_per_s": round(closed / t_per, 1),
                    "gain_vs_per_request": round(t_per / t_srv, 3),
                    "service_GBps_alg": round(balg / t_srv / 1e9, 1),
-                   "max_rel_l2_vs_fp64_sample": max(errs), "service_steps": steps["steps"]})
+                   "max_rel_l2_vs_fp64_sample": max(errs), "service_launches": stats["launches"],
+                   "lanes": lanes})
     log(f"serving closed loop: service {closed / t_srv:.0f} solves/s, per-request path {closed / t_per:.0f} "
         f"solves/s, gain {t_per / t_srv:.2f}x, max err {max(errs):.2e}")
     return result
 
 
-def load_traffic():
+def load_traffic(kernel_tag):
+    """HBM bytes per launch of the given kernel from the committed counter
+    summary (FETCH_SIZE and WRITE_SIZE scaled by the copy probe), or None."""
     p = os.path.join(ROOT, "profiles", "traffic_pass_kernel_512cubed_fp32.json")
     if not os.path.exists(p):
         return None
     with open(p) as f:
-        return json.load(f).get("traffic_bytes_per_launch")
+        d = json.load(f)
+    per = [e["fetch_bytes"] + e["write_bytes"] for e in d.get("per_launch", []) if kernel_tag in e["kernel"]]
+    return sum(per) / len(per) if per else None
+
+
+def fp64_roundtrip(P, torch, n, steps=5):
+    """The FP64 drop-in path on the same grid and layouts (extras)."""
+    pg = P.init_proc_grid((1, 1, 1))
+    g_real = P.init_data_grid((n, n, n), -1, pg, (0, 1, 2), (2, 1, 0))
+    g_spec = P.init_data_grid((n // 2 + 1, n, n), 0, pg, (0, 1, 2), (1, 2, 0))
+    pf = P.plan_3Dtrans(g_real, g_spec, P.init_3Dtype((P.type_id("R2CFFT_D"), P.type_id("CFFT_FORWARD_D"),
+                                                        P.type_id("CFFT_FORWARD_D"))))
+    pb = P.plan_3Dtrans(g_spec, g_real, P.init_3Dtype((P.type_id("C2RFFT_D"), P.type_id("CFFT_BACKWARD_D"),
+                                                        P.type_id("CFFT_BACKWARD_D"))))
+    f = torch.rand((n, n, n), device="cuda", dtype=torch.float64, generator=torch.Generator(device="cuda").manual_seed(3))
+    spec = torch.empty((n, n // 2 + 1, n), dtype=torch.complex128, device="cuda")
+    back = torch.empty_like(f)
+    st = P.stream()
+    torch.cuda.synchronize()
+    for i in range(steps + 1):
+        if i == 1:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+        P.exec_async(pf, f.data_ptr(), spec.data_ptr(), -1, 0, st, prec=8)
+        P.exec_async(pb, spec.data_ptr(), back.data_ptr(), -1, 0, st, prec=8)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / steps
+    err = (torch.linalg.vector_norm(back / n ** 3 - f) / torch.linalg.vector_norm(f)).item()
+    log(f"fp64 {n}^3 round trip {t * 1e3:.3f} ms, rel L2 {err:.2e}")
+    res = {"ms_per_roundtrip": round(t * 1e3, 4), "GBps_alg": round(b_alg(n, 8) / t / 1e9, 1),
+           "pct_hbm_peak": round(100 * b_alg(n, 8) / t / 1e9 / HBM_PEAK_GBS, 2), "roundtrip_rel_l2": err}
+    del f, spec, back
+    torch.cuda.empty_cache()
+    return res
 
 
 def main():
@@ -279,6 +331,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-item batch measurement")
     ap.add_argument("--no-serving", action="store_true", help="skip the Poisson service measurement")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 round trip (extras)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed steps, run a plain 256 MiB device copy three times (the "
                          "known-size kernel that scales HBM byte counters under rocprofv3 --pmc)")
@@ -343,10 +396,12 @@ def main():
     t_step = t / args.steps
     balg = b_alg(n, 4)
     value = balg / t_step / 1e9
-    launches = nf * len(avg_f) + nb * len(avg_b)
-    avg_launch_ms = (tot_f + tot_b) / max(launches, 1)
-    per_launch_bytes = balg / (len(avg_f) + len(avg_b))
-    achieved = per_launch_bytes / (avg_launch_ms * 1e-3) / 1e9
+    # dominant kernel: pass_kernel_pow2<float, 512>, the four C2C passes of a
+    # step (forward y, z; backward y, z), each reading and writing V_c
+    vc = 2 * 4 * (n // 2 + 1) * n * n
+    c2c_ms = [avg_f[1], avg_f[2], avg_b[0], avg_b[1]]
+    c2c_avg = sum(c2c_ms) / 4
+    achieved = 2 * vc / (c2c_avg * 1e-3) / 1e9
     log(f"step {t_step * 1e3:.3f} ms  {value:.1f} GB/s  roundtrip rel L2 {err:.2e}")
     log("forward pass avg ms:", [round(v, 4) for v in avg_f], "backward:", [round(v, 4) for v in avg_b])
     if args.copy_probe:
@@ -356,11 +411,14 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         log("copy probe: 3 x 268435456-byte device copies done")
-    traffic = load_traffic()
+    traffic = load_traffic("pow2<float, 512>")
+    del f, spec, back
+    torch.cuda.empty_cache()
+    fp64 = None if args.no_fp64 else fp64_roundtrip(P, torch, n)
     cpu = None
     if not args.no_cpu_baseline:
-        log("cpu baseline (oracle, 256^3 sample) ...")
-        cpu = cpu_baseline()
+        log(f"cpu baseline (oracle, {n}^3 FP64 round trip, one thread) ...")
+        cpu = cpu_baseline(n)
     batch = None if args.no_batch else batch_bench(P, torch)
     serving = None if args.no_serving else serving_bench(P, torch)
     out = {
@@ -382,11 +440,13 @@ def main():
                                "sin(x)sin(2y)sin(3z)+N(0,1e-3)",
                    "grid": [n, n, n], "types": "R2CFFT_S,CFFT_FORWARD_S,CFFT_FORWARD_S / back",
                    "b_alg_bytes": balg, "parallelism": "1 GPU, 1x1 processor grid"},
-        "roofline": {"kernel": "p3e::pass_kernel_pow2<float,M> (M=512 C2C, M=256 R2C/C2R; all 6 stage launches)", "bound": "hbm", "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "bytes_per_launch_alg": per_launch_bytes,
-                     "avg_launch_ms": round(avg_launch_ms, 5)},
+        "roofline": {"kernel": "p3e::pass_kernel_pow2<float, 512> (the 4 C2C passes of a step)", "bound": "hbm",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "bytes_per_launch_alg": 2 * vc, "avg_launch_ms": round(c2c_avg, 5),
+                     "per_pass_ms": {"fwd": [round(v, 4) for v in avg_f], "bwd": [round(v, 4) for v in avg_b]}},
         "cpu_baseline": cpu,
+        "fp64_roundtrip": fp64,
         "batch": batch,
         "serving": serving,
     }

@@ -177,3 +177,23 @@ def test_service_large_grid_dedicated_launches():
     assert err < TOL[4]
     for s, (d, seed) in zip(mix, reqs):
         assert C.rel_l2(s, oracle(d, seed)) < TOL[4]
+
+
+def test_large_edges_vs_oracle():
+    """Edges at the top of the serving workload's range, with the largest
+    in-register prime radices (344 = 8 * 43, 376 = 8 * 47, 368 = 16 * 23):
+    the per-request path and the service against the oracle's solve, and the
+    two paths give the same bits (same plans, same kernels)."""
+    d = [344, 376, 368]
+    log("case: poisson large edges", d)
+    _, t = rhs_of(d, 40, 4)
+    out = torch.empty_like(t)
+    torch.cuda.synchronize()
+    P.poisson_solve(d, t.data_ptr(), out.data_ptr(), 4)
+    ref = oracle(d, 40)
+    e1 = C.rel_l2(out.cpu().numpy(), ref)
+    sols, _, _ = run_server([(d, 40), (SIZES[0], 41)], 4)
+    e2 = C.rel_l2(sols[0], ref)
+    log("  rel L2 vs oracle: per-request", e1, "service", e2)
+    assert e1 < TOL[4] and e2 < TOL[4]
+    assert np.array_equal(sols[0], out.cpu().numpy())
