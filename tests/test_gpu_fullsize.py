@@ -150,12 +150,13 @@ def test_config3_full_size_lines_vs_oracle(prec):
     desc = P.describe(h3)
     passes = _passes(desc)
     log("case: 768x640x513 stages, prec", prec, ":", desc)
-    assert [p[1] for p in passes] == [0, 1, 2], desc
+    assert sorted(p[1] for p in passes) == [0, 1, 2], desc
     pg = P.init_proc_grid((1, 1, 1))
     rng = np.random.default_rng(30 + prec)
     gen = torch.Generator(device="cuda").manual_seed(prec)
     rdt, cdt = (torch.float64, torch.complex128) if prec == 8 else (torch.float32, torch.complex64)
-    for (kind, axis, N, moi, moo), tid, tid_d in zip(passes, types, types_d):
+    for kind, axis, N, moi, moo in passes:
+        tid, tid_d = types[axis], types_d[axis]
         din_dims = [dims[0] if axis == 0 else nh, dims[1], dims[2]]
         dout_dims = [nh, dims[1], dims[2]]
         g1 = P.init_data_grid(din_dims, -1, pg, (0, 1, 2), moi)

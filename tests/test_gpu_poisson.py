@@ -96,12 +96,26 @@ def test_service_mixed_sizes_vs_oracle(prec):
     reqs = [(SIZES[i % len(SIZES)], 1) for i in range(12)]
     log("case: service", prec, len(reqs), "requests")
     sols, recs, st = run_server(reqs, prec)
-    log("  steps", st)
+    log("  stats", st)
     errs = [C.rel_l2(s, oracle(d, seed)) for s, (d, seed) in zip(sols, reqs)]
     log("  max rel L2", max(errs))
     assert max(errs) < TOL[prec]
-    # the 12 requests were submitted together: they shared steps
-    assert st["steps"] < 12 * 6
+    # six passes per request, whether a request ran alone or in a group
+    assert 0 < st["launches"] <= 12 * 6
+
+
+def test_service_groups_small_requests():
+    """Small grids submitted back to back share step-kernel launches (six
+    per group instead of six per request), with every result the same as
+    the request alone."""
+    d = [48, 40, 36]
+    alone, _, _ = run_server([(d, 5)], 4)
+    reqs = [(d, 5)] * 32
+    sols, _, st = run_server(reqs, 4)
+    log("  stats", st)
+    assert st["launches"] < 32 * 6
+    for s in sols:
+        assert np.array_equal(s, alone[0])
 
 
 def test_service_alone_equals_mix():
