@@ -322,6 +322,58 @@ def fp64_roundtrip(P, torch, n, steps=5):
     return res
 
 
+def mixed_bench(P, torch, steps=5):
+    """768 x 640 x 513 real grid, FP32: R2C x, C2C y, DCT1_COMPLEX z forward
+    and back (memory order (0,1,2) on both sides), field of Chebyshev series
+    with coefficients decaying as 1/(k+1)^2; per-pass HIP-event times."""
+    import cases as C
+    dims = [768, 640, 513]
+    nx, ny, nz = dims
+    pg = P.init_proc_grid((1, 1, 1))
+    g_real = P.init_data_grid(dims, -1, pg, (0, 1, 2), (0, 1, 2))
+    g_spec = P.init_data_grid([nx // 2 + 1, ny, nz], 0, pg, (0, 1, 2), (0, 1, 2))
+    pf = P.plan_3Dtrans(g_real, g_spec, P.init_3Dtype((P.type_id("R2CFFT_S"), P.type_id("CFFT_FORWARD_S"),
+                                                        P.type_id("DCT1_COMPLEX_S"))))
+    pb = P.plan_3Dtrans(g_spec, g_real, P.init_3Dtype((P.type_id("C2RFFT_S"), P.type_id("CFFT_BACKWARD_S"),
+                                                        P.type_id("DCT1_COMPLEX_S"))))
+    x = C.chebyshev_field(torch, dims, 3).float()
+    spec = torch.empty((nz, ny, nx // 2 + 1), dtype=torch.complex64, device="cuda")
+    back = torch.empty_like(x)
+    st = P.stream()
+    torch.cuda.synchronize()
+    P.exec_async(pf, x.data_ptr(), spec.data_ptr(), -1, 0, st, prec=4)
+    P.exec_async(pb, spec.data_ptr(), back.data_ptr(), -1, 0, st, prec=4)
+    torch.cuda.synchronize()
+    P.profile(pf, True)
+    P.profile(pb, True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        P.exec_async(pf, x.data_ptr(), spec.data_ptr(), -1, 0, st, prec=4)
+        P.exec_async(pb, spec.data_ptr(), back.data_ptr(), -1, 0, st, prec=4)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / steps
+    _, avg_f, _ = P.profile_read(pf)
+    _, avg_b, _ = P.profile_read(pb)
+    P.profile(pf, False)
+    P.profile(pb, False)
+    scale = 1.0 / (nx * ny * 2 * (nz - 1))
+    err = (torch.linalg.vector_norm(back.double() * scale - x.double()) / torch.linalg.vector_norm(x.double())).item()
+    vr = 4 * nx * ny * nz
+    vc = 8 * (nx // 2 + 1) * ny * nz
+    balg = 2 * (vr + 5 * vc)
+    log(f"mixed 768x640x513 FP32 round trip {t * 1e3:.3f} ms ({balg / t / 1e9:.0f} GB/s), rel L2 {err:.2e}; "
+        f"passes fwd {[round(v, 4) for v in avg_f]} bwd {[round(v, 4) for v in avg_b]}; "
+        f"{P.describe(pf)} || {P.describe(pb)}")
+    res = {"workload": "768x640x513 FP32 R2C x / C2C y / DCT1_COMPLEX z, forward and back, mem order (0,1,2)",
+           "ms_per_roundtrip": round(t * 1e3, 4), "GBps_alg": round(balg / t / 1e9, 1),
+           "pct_hbm_peak": round(100 * balg / t / 1e9 / HBM_PEAK_GBS, 2), "roundtrip_rel_l2": err,
+           "per_pass_ms": {"fwd": [round(v, 4) for v in avg_f], "bwd": [round(v, 4) for v in avg_b]},
+           "b_alg_bytes": balg}
+    del x, spec, back
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -332,6 +384,10 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-item batch measurement")
     ap.add_argument("--no-serving", action="store_true", help="skip the Poisson service measurement")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 round trip (extras)")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the 768x640x513 mixed-type round trip (extras)")
+    ap.add_argument("--only", choices=["roundtrip", "mixed", "batch", "serving"],
+                    help="run one workload alone and print its result as one JSON line (for profiling); "
+                         "'serving' runs the closed loop only")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed steps, run a plain 256 MiB device copy three times (the "
                          "known-size kernel that scales HBM byte counters under rocprofv3 --pmc)")
@@ -348,6 +404,17 @@ def main():
     assert torch.cuda.is_available()
     torch.cuda.set_device(0)
     P.setup()
+    if args.only in ("mixed", "batch", "serving"):
+        if args.only == "mixed":
+            res = mixed_bench(P, torch, args.steps)
+        elif args.only == "batch":
+            res = batch_bench(P, torch)
+        else:
+            res = serving_bench(P, torch, rates=(), seconds=())
+        print(json.dumps(res), flush=True)
+        return
+    if args.only == "roundtrip":
+        args.no_cpu_baseline = args.no_batch = args.no_serving = args.no_fp64 = args.no_mixed = True
     pg = P.init_proc_grid((1, 1, 1))
     mo1, mo2 = (2, 1, 0), (1, 2, 0)
     g_real = P.init_data_grid((n, n, n), -1, pg, (0, 1, 2), mo1)
@@ -415,6 +482,7 @@ def main():
     del f, spec, back
     torch.cuda.empty_cache()
     fp64 = None if args.no_fp64 else fp64_roundtrip(P, torch, n)
+    mixed = None if args.no_mixed else mixed_bench(P, torch)
     cpu = None
     if not args.no_cpu_baseline:
         log(f"cpu baseline (oracle, {n}^3 FP64 round trip, one thread) ...")
@@ -447,6 +515,7 @@ def main():
                      "per_pass_ms": {"fwd": [round(v, 4) for v in avg_f], "bwd": [round(v, 4) for v in avg_b]}},
         "cpu_baseline": cpu,
         "fp64_roundtrip": fp64,
+        "mixed": mixed,
         "batch": batch,
         "serving": serving,
     }

@@ -81,3 +81,30 @@ def gaussians_rhs(dims, seed, count=8):
         g = [np.exp(-(np.angle(np.exp(1j * (ax - ci))) ** 2) / (2 * w * w)) for ax, ci in zip(axes, c)]
         f += a * g[0][:, None, None] * g[1][None, :, None] * g[2][None, None, :]
     return f - f.mean()
+
+
+def chebyshev_field(torch, dims, seed):
+    """Real field on the periodic x, y grid and the Chebyshev-Gauss-Lobatto z
+    grid: a sum of 4 separable terms, each axis a series with coefficients
+    decaying as 1/(k+1)^2 (uniform phases in x and y, random signs in z);
+    storage order (0,1,2) ([z][y][x]), FP64, on the GPU."""
+    nx, ny, nz = dims
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    dev, f64 = "cuda", torch.float64
+
+    def periodic(n, kmax):
+        k = torch.arange(kmax, device=dev, dtype=f64)
+        ph = torch.rand(kmax, generator=g, device=dev, dtype=f64) * 2 * torch.pi
+        x = torch.arange(n, device=dev, dtype=f64) * (2 * torch.pi / n)
+        return (torch.cos(k[:, None] * x[None, :] + ph[:, None]) / (k[:, None] + 1) ** 2).sum(0)
+
+    def cheb(n):
+        k = torch.arange(n, device=dev, dtype=f64)
+        sg = torch.where(torch.rand(n, generator=g, device=dev) < 0.5, -1.0, 1.0).to(f64)
+        j = torch.arange(n, device=dev, dtype=f64)
+        return (torch.cos(torch.pi * k[:, None] * j[None, :] / (n - 1)) * (sg / (k + 1) ** 2)[:, None]).sum(0)
+
+    f = torch.zeros((nz, ny, nx), device=dev, dtype=f64)
+    for _ in range(4):
+        f += cheb(nz)[:, None, None] * periodic(ny, 64)[None, :, None] * periodic(nx, 64)[None, None, :]
+    return f

@@ -39,33 +39,6 @@ def lib_setup():
     torch.cuda.synchronize()
 
 
-def chebyshev_field(dims, seed):
-    """Real field on the periodic x, y grid and the Chebyshev-Gauss-Lobatto z
-    grid: a sum of 4 separable terms, each axis a series with coefficients
-    decaying as 1/(k+1)^2 (uniform phases in x and y, random signs in z);
-    storage order (0,1,2) ([z][y][x]), FP64, on the GPU."""
-    nx, ny, nz = dims
-    g = torch.Generator(device="cuda").manual_seed(seed)
-    dev, f64 = "cuda", torch.float64
-
-    def periodic(n, kmax):
-        k = torch.arange(kmax, device=dev, dtype=f64)
-        ph = torch.rand(kmax, generator=g, device=dev, dtype=f64) * 2 * torch.pi
-        x = torch.arange(n, device=dev, dtype=f64) * (2 * torch.pi / n)
-        return (torch.cos(k[:, None] * x[None, :] + ph[:, None]) / (k[:, None] + 1) ** 2).sum(0)
-
-    def cheb(n):
-        k = torch.arange(n, device=dev, dtype=f64)
-        sg = torch.where(torch.rand(n, generator=g, device=dev) < 0.5, -1.0, 1.0).to(f64)
-        j = torch.arange(n, device=dev, dtype=f64)
-        return (torch.cos(torch.pi * k[:, None] * j[None, :] / (n - 1)) * (sg / (k + 1) ** 2)[:, None]).sum(0)
-
-    f = torch.zeros((nz, ny, nx), device=dev, dtype=f64)
-    for _ in range(4):
-        f += cheb(nz)[:, None, None] * periodic(ny, 64)[None, :, None] * periodic(nx, 64)[None, None, :]
-    return f
-
-
 def spectrum_entry(f_host, kx, ky, kz):
     """FP64 spectrum entry from the definitions: FFTW R2C (x), forward DFT
     (y), REDFT00 (z): sum_x sum_y sum_z f e^{-2 pi i kx x/nx} e^{-2 pi i ky y/ny}
@@ -114,7 +87,7 @@ def run(prec, f64):
 
 def test_config3_full_size():
     log("case: 768x640x513 R2C/C2C/DCT-I, FP64 and FP32")
-    f = chebyshev_field(DIMS, 3)
+    f = C.chebyshev_field(torch, DIMS, 3)
     scale = 1.0 / (768 * 640 * 2 * 512)
     spec64, back64 = run(8, f)
     rt64 = (torch.linalg.vector_norm(back64 * scale - f) / torch.linalg.vector_norm(f)).item()
