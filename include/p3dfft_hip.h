@@ -25,6 +25,22 @@ void p3dfft_hip_clear_error(void);
 int p3dfft_exec_3Dtrans_double_async(int plan, double *in, double *out, int idir, int OW, void *stream);
 int p3dfft_exec_3Dtrans_single_async(int plan, float *in, float *out, int idir, int OW, void *stream);
 
+/* Exec of a 3D plan that allocates nothing (no hipMalloc, hipFree, stream
+ * or event creation, no host heap): a variant of the *_async calls above
+ * with no counterpart in the reference, for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation).  The precision is the plan's; in and out must be device
+ * arrays; stream NULL is the null stream.  The intermediate arrays of the
+ * plan's passes live in one fixed device array of the library,
+ * P3DFFT_HIP_NOALLOC_SCRATCH_BYTES long, held by each device that runs this
+ * call until the process ends; a plan needing more is refused with an
+ * error.  Every call on one device uses the same array, so calls that
+ * overlap on a device (other streams, other threads) race.  No timing or
+ * TIMERS phases are recorded.  Returns 0, or -1 with the reason in
+ * p3dfft_hip_last_error(). */
+#define P3DFFT_HIP_NOALLOC_SCRATCH_BYTES (64 << 20)
+int p3dfft_hip_exec_3Dtrans_noalloc(int plan, void *in, void *out, int idir, int OW, void *stream);
+
 /* The library's own stream (created on first use). */
 void *p3dfft_hip_stream(void);
 

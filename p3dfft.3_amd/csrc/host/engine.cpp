@@ -143,6 +143,8 @@ Plan *plan_from_chain(int prec, PassChain &chain, const Layout &in, const Layout
   p->prof = false;
   p->ev_used = 0;
   for (int a = 0; a < 3; a++) p->pass_of_axis[a] = chain.pass_of_axis[a];
+  for (int al = 0; al < 2; al++)
+    assign_buffers(chain.passes, p->out_bytes, al == 1, p->buf_dst[al], p->buf_need[al], &p->buf_copy_back[al]);
   for (PassDesc &d : chain.passes) {
     Pass ps;
     ps.d = d;
@@ -194,6 +196,47 @@ static int ensure(void **buf, size_t *cap, size_t need) {
   return 0;
 }
 
+// Arguments of pass ps reading src and writing dst; the derivative is
+// applied by the pass that transforms axis idir.
+static PassArgs pass_args(const Pass &ps, const void *src, void *dst, int idir) {
+  PassArgs a = ps.d.args;
+  a.in = src;
+  a.out = dst;
+  a.deriv = (idir >= 0 && idir == ps.d.axis && ps.d.deriv_capable) ? 1 : 0;
+  return a;
+}
+
+int plan_launch(Plan *p, const void *din, void *dout, void *const scr[2], int idir, hipStream_t st) {
+  const bool alias = (din == (const void *)dout);
+  if (p->copy_only) {
+    if (!alias) HIPCHK(hipMemcpyAsync(dout, din, p->out_bytes, hipMemcpyDeviceToDevice, st));
+    return 0;
+  }
+  const int *dst = p->buf_dst[alias];
+  const void *src = din;
+  for (size_t i = 0; i < p->passes.size(); i++) {
+    const Pass &ps = p->passes[i];
+    void *d = dst[i] == BUF_OUT ? dout : scr[dst[i] == BUF_S0 ? 0 : 1];
+    const PassArgs a = pass_args(ps, src, d, idir);
+    const int rc = p3e_launch_pass(&a, p->prec, ps.d.lds_bytes, ps.d.threads, ps.d.nblocks, st);
+    if (rc) {
+      set_error("pass kernel launch failed: %s", hipGetErrorString((hipError_t)rc));
+      return -1;
+    }
+    src = d;
+  }
+  if (p->buf_copy_back[alias]) HIPCHK(hipMemcpyAsync(dout, src, p->out_bytes, hipMemcpyDeviceToDevice, st));
+  return 0;
+}
+
+// End of an exec: the copy of a staged result to the caller's host array,
+// and the wait of a synchronous call.
+static int finish_exec(Plan *p, void *out, void *dout, bool out_host, bool wait, hipStream_t st) {
+  if (out_host) HIPCHK(hipMemcpyAsync(out, dout, p->out_bytes, hipMemcpyDeviceToHost, st));
+  if (wait) HIPCHK(hipStreamSynchronize(st));
+  return 0;
+}
+
 int plan_exec(Plan *p, const void *in, void *out, int idir, bool OW, hipStream_t st, bool sync) {
   (void)OW;  // the user input is never written, so OW only relaxes checks
   if (!p) {
@@ -235,14 +278,15 @@ int plan_exec(Plan *p, const void *in, void *out, int idir, bool OW, hipStream_t
       tm_phase[tm_n++] = PH_TRANS_EXEC;  // build/exec.C:546-574: the memcpy of an EMPTY stage
     }
   } else {
-    std::vector<PassDesc> descs;
-    for (Pass &ps : p->passes) descs.push_back(ps.d);
-    int dst[8];
-    size_t need[2];
-    bool copy_back = false;
-    assign_buffers(descs, p->out_bytes, alias, dst, need, &copy_back);
+    const int *dst = p->buf_dst[alias];
+    const size_t *need = p->buf_need[alias];
+    const bool copy_back = p->buf_copy_back[alias];
     for (int s = 0; s < 2; s++)
       if (need[s] && ensure(&p->scratch[s], &p->scratch_cap[s], need[s])) return -1;
+    if (!p->prof && !tm) {
+      if (plan_launch(p, din, dout, p->scratch, idir, st)) return -1;
+      return finish_exec(p, out, dout, out_host, sync || in_host || out_host, st);
+    }
     const void *src = din;
     for (size_t i = 0; i < p->passes.size(); i++) {
       Pass &ps = p->passes[i];
@@ -252,10 +296,7 @@ int plan_exec(Plan *p, const void *in, void *out, int idir, bool OW, hipStream_t
         case BUF_S0: d = p->scratch[0]; break;
         default: d = p->scratch[1]; break;
       }
-      PassArgs a = ps.d.args;
-      a.in = src;
-      a.out = d;
-      a.deriv = (idir >= 0 && idir == ps.d.axis && ps.d.deriv_capable) ? 1 : 0;
+      PassArgs a = pass_args(ps, src, d, idir);
       hipEvent_t e0 = nullptr, e1 = nullptr;
       if (p->prof) {
         while (p->ev_pool.size() < p->ev_used + 2) {
@@ -300,8 +341,7 @@ int plan_exec(Plan *p, const void *in, void *out, int idir, bool OW, hipStream_t
     }
     if (copy_back) HIPCHK(hipMemcpyAsync(dout, src, p->out_bytes, hipMemcpyDeviceToDevice, st));
   }
-  if (out_host) HIPCHK(hipMemcpyAsync(out, dout, p->out_bytes, hipMemcpyDeviceToHost, st));
-  if (sync || in_host || out_host) HIPCHK(hipStreamSynchronize(st));
+  if (finish_exec(p, out, dout, out_host, sync || in_host || out_host, st)) return -1;
   for (int i = 0; i < tm_n; i++) {
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, p->tm_ev[2 * i], p->tm_ev[2 * i + 1]));

@@ -37,6 +37,11 @@ struct Plan {
   std::vector<Pass> passes;
   int pass_of_axis[3];
   bool copy_only;
+  // destination buffer of each pass, scratch sizes and the final copy,
+  // for separate (index 0) and aliased (index 1) in and out arrays
+  int buf_dst[2][8];
+  size_t buf_need[2][2];
+  bool buf_copy_back[2];
   void *scratch[2];
   size_t scratch_cap[2];
   void *stage_in, *stage_out;
@@ -65,6 +70,11 @@ Plan *plan_from_chain(int prec, PassChain &chain, const Layout &in, const Layout
 // idir: logical axis of the spectral derivative, or -1.  in/out may be host
 // or device pointers; host arrays are staged and imply a synchronous call.
 int plan_exec(Plan *p, const void *in, void *out, int idir, bool OW, hipStream_t st, bool sync);
+// The passes of p from device array din to device array dout on st, with
+// the intermediate arrays at scr[0] and scr[1] (at least p->buf_need bytes
+// each for the aliasing case of din and dout); asynchronous, allocates
+// nothing, records no timing.
+int plan_launch(Plan *p, const void *din, void *dout, void *const scr[2], int idir, hipStream_t st);
 void plan_destroy(Plan *p);
 
 // Standalone derivative of a stored spectrum (compute_deriv semantics).

@@ -59,6 +59,7 @@ def lib():
             getattr(L, f"p3dfft_compute_deriv_{n}").argtypes = [VP, VP, GridP, ctypes.c_int]
             getattr(L, f"p3dfft_exec_3Dtrans_{n}_async").argtypes = [ctypes.c_int, VP, VP, ctypes.c_int,
                                                                      ctypes.c_int, VP]
+        L.p3dfft_hip_exec_3Dtrans_noalloc.argtypes = [ctypes.c_int, VP, VP, ctypes.c_int, ctypes.c_int, VP]
         L.p3dfft_hip_last_error.argtypes = [ctypes.c_char_p, ctypes.c_int]
         L.p3dfft_hip_plan_describe.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
         L.p3dfft_hip_pass_bytes.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong),
@@ -165,6 +166,15 @@ def exec_async(plan, inp, out, idir=-1, OW=0, stream=None, prec=8):
     if rc:
         check()
         raise RuntimeError("async exec failed")
+
+
+def exec_noalloc(plan, inp, out, idir=-1, OW=0, stream=None):
+    """Exec of a 3D plan on device arrays whose intermediate arrays live in
+    the library's fixed device scratch array (allocates nothing)."""
+    rc = lib().p3dfft_hip_exec_3Dtrans_noalloc(plan, inp, out, idir, OW, stream)
+    if rc:
+        _, msg = last_error()
+        raise RuntimeError("no-alloc exec failed: " + msg)
 
 
 def describe(plan):
