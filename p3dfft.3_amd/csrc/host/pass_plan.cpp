@@ -293,6 +293,14 @@ static void setup_fast(int prec, const OpSpec &op, PassDesc &ps, int batch_threa
   auto lds = [&](int b) { return (size_t)b * lp * esz + tables; };
   while (B * 2 <= Bmax && lds(B * 2) <= LDS_BUDGET && (long)M * B * 2 / P3E_FAST_E <= maxth) B *= 2;
   if (lds(B) > LDS_MAX || (long)M * B / P3E_FAST_E > maxth) return;
+  // the kernel addresses a tile's elements with 32-bit byte offsets from
+  // the tile's first element: the largest offset within a tile (one element
+  // past it, for paired accesses) must stay below 2^31 on both sides
+  const double in_esz = (double)prec * (op.cls == CLS_R2C ? 1 : 2);
+  const double out_esz = (double)prec * (op.cls == CLS_C2R ? 1 : 2);
+  const double span_in = ((double)a.nin * a.is_d + (double)B * a.is_e + 1) * in_esz;
+  const double span_out = ((double)a.nout * a.os_d + (double)B * a.os_e + 1) * out_esz;
+  if (span_in >= 2147483648.0 || span_out >= 2147483648.0) return;
   a.fast = 1;
   a.mid = 0;
   a.mid_prime = 0;

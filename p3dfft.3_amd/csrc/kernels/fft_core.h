@@ -12,6 +12,10 @@
 #define P3E_HD
 #define P3E_INL inline
 #endif
+// a lambda whose body must be inlined into its caller (a lambda passed to a
+// phase runner that is not inlined would take the kernel's arguments off
+// registers into scratch memory)
+#define P3E_LAMBDA_INL __attribute__((always_inline))
 
 #include "pass_args.h"
 

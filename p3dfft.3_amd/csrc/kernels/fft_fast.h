@@ -24,8 +24,12 @@ template <> struct Log2<1> {
   static constexpr int v = 0;
 };
 
-// Radix-R Stockham pass at stride Ns of a length-M FFT, split in halves.
-template <typename T, int R, int M, int Ns> struct FPass {
+// Radix-R Stockham pass at stride Ns of a length-M FFT with DFT sign S,
+// split in halves.  Every thread owns NB butterflies of the B lines of the
+// tile (the planner makes nth * E = B * M), including lines past the end
+// of a partial tile: their LDS slots hold stale values, transformed and
+// never stored, so no butterfly needs a bounds check.
+template <typename T, int R, int M, int Ns, int S> struct FPass {
   static constexpr int E = P3E_FAST_E;
   static constexpr int SLOTS = E;  // registers per thread between the halves
   static constexpr int NB = E / R;  // butterflies per thread
@@ -33,42 +37,36 @@ template <typename T, int R, int M, int Ns> struct FPass {
   static constexpr int lognbf = Log2<M>::v - logR;
   static constexpr int nbf = M / R;
   cx<T> *buf;
-  int lp, nb, s;
+  int lp;
   const cx<T> *tw;
 
   P3E_HD P3E_INL void read(int tid, int nth, cx<T> *v) const {
-    const int total = nb << lognbf;
 #pragma unroll
     for (int m = 0; m < NB; m++) {
       const int b = tid + m * nth;
-      if (b < total) {
-        const cx<T> *sl = buf + (b >> lognbf) * lp;
-        const int j = b & (nbf - 1);
+      const cx<T> *sl = buf + (b >> lognbf) * lp;
+      const int j = b & (nbf - 1);
 #pragma unroll
-        for (int r = 0; r < R; r++) v[m * R + r] = sl[P3E_PAD(j + r * nbf)];
-      }
+      for (int r = 0; r < R; r++) v[m * R + r] = sl[P3E_PAD(j + r * nbf)];
     }
   }
 
   P3E_HD P3E_INL void write(int tid, int nth, cx<T> *v) const {
-    const int total = nb << lognbf;
 #pragma unroll
     for (int m = 0; m < NB; m++) {
       const int b = tid + m * nth;
-      if (b < total) {
-        const int j = b & (nbf - 1);
-        const int k = j & (Ns - 1);
-        cx<T> *w = v + m * R;
-        if (Ns > 1) {
+      const int j = b & (nbf - 1);
+      const int k = j & (Ns - 1);
+      cx<T> *w = v + m * R;
+      if (Ns > 1) {
 #pragma unroll
-          for (int r = 1; r < R; r++) w[r] = mul(w[r], twid(tw, k * tw_stride(R) + r - 1, s));
-        }
-        bfly<T, R>(w, s);
-        cx<T> *dl = buf + (b >> lognbf) * lp;
-        const int base = ((j - k) << logR) + k;
-#pragma unroll
-        for (int r = 0; r < R; r++) dl[P3E_PAD(base + r * Ns)] = w[r];
+        for (int r = 1; r < R; r++) w[r] = mul(w[r], twid(tw, k * tw_stride(R) + r - 1, S));
       }
+      bfly<T, R>(w, S);
+      cx<T> *dl = buf + (b >> lognbf) * lp;
+      const int base = ((j - k) << logR) + k;
+#pragma unroll
+      for (int r = 0; r < R; r++) dl[P3E_PAD(base + r * Ns)] = w[r];
     }
   }
 };
@@ -77,25 +75,25 @@ template <typename T, int R, int M, int Ns> struct FPass {
 // one pass (read half, barrier, write half, barrier).  tw: the twiddles of
 // this pass onward, pass by pass (pass_twiddles on the host): Ns
 // tw_stride(R) entries for a radix-R pass at stride Ns.
-template <typename T, int M, int Ns, typename Run, int R, int... Rest>
-P3E_HD P3E_INL void fft_seq(Run &run, cx<T> *buf, int lp, int nb, int s, const cx<T> *tw) {
-  FPass<T, R, M, Ns> ps{buf, lp, nb, s, tw};
+template <typename T, int M, int Ns, int S, typename Run, int R, int... Rest>
+P3E_HD P3E_INL void fft_seq(Run &run, cx<T> *buf, int lp, const cx<T> *tw) {
+  FPass<T, R, M, Ns, S> ps{buf, lp, tw};
   run(ps);
-  if constexpr (sizeof...(Rest) > 0) fft_seq<T, M, Ns * R, Run, Rest...>(run, buf, lp, nb, s, tw + Ns * tw_stride(R));
+  if constexpr (sizeof...(Rest) > 0) fft_seq<T, M, Ns * R, S, Run, Rest...>(run, buf, lp, tw + Ns * tw_stride(R));
 }
 
 // Radix sequence of each supported length (all radices divide P3E_FAST_E).
-template <typename T, int M, typename Run>
-P3E_HD P3E_INL void fft_pow2(Run &run, cx<T> *buf, int lp, int nb, int s, const cx<T> *tw) {
-  if constexpr (M == 16) fft_seq<T, M, 1, Run, 16>(run, buf, lp, nb, s, tw);
-  else if constexpr (M == 32) fft_seq<T, M, 1, Run, 8, 4>(run, buf, lp, nb, s, tw);
-  else if constexpr (M == 64) fft_seq<T, M, 1, Run, 8, 8>(run, buf, lp, nb, s, tw);
-  else if constexpr (M == 128) fft_seq<T, M, 1, Run, 16, 8>(run, buf, lp, nb, s, tw);
-  else if constexpr (M == 256) fft_seq<T, M, 1, Run, 16, 16>(run, buf, lp, nb, s, tw);
-  else if constexpr (M == 512) fft_seq<T, M, 1, Run, 8, 8, 8>(run, buf, lp, nb, s, tw);
-  else if constexpr (M == 1024) fft_seq<T, M, 1, Run, 16, 8, 8>(run, buf, lp, nb, s, tw);
-  else if constexpr (M == 2048) fft_seq<T, M, 1, Run, 16, 16, 8>(run, buf, lp, nb, s, tw);
-  else fft_seq<T, M, 1, Run, 16, 16, 16>(run, buf, lp, nb, s, tw);
+template <typename T, int M, int S, typename Run>
+P3E_HD P3E_INL void fft_pow2(Run &run, cx<T> *buf, int lp, const cx<T> *tw) {
+  if constexpr (M == 16) fft_seq<T, M, 1, S, Run, 16>(run, buf, lp, tw);
+  else if constexpr (M == 32) fft_seq<T, M, 1, S, Run, 8, 4>(run, buf, lp, tw);
+  else if constexpr (M == 64) fft_seq<T, M, 1, S, Run, 8, 8>(run, buf, lp, tw);
+  else if constexpr (M == 128) fft_seq<T, M, 1, S, Run, 16, 8>(run, buf, lp, tw);
+  else if constexpr (M == 256) fft_seq<T, M, 1, S, Run, 16, 16>(run, buf, lp, tw);
+  else if constexpr (M == 512) fft_seq<T, M, 1, S, Run, 8, 8, 8>(run, buf, lp, tw);
+  else if constexpr (M == 1024) fft_seq<T, M, 1, S, Run, 16, 8, 8>(run, buf, lp, tw);
+  else if constexpr (M == 2048) fft_seq<T, M, 1, S, Run, 16, 16, 8>(run, buf, lp, tw);
+  else fft_seq<T, M, 1, S, Run, 16, 16, 16>(run, buf, lp, tw);
 }
 
 // Tile coordinates of workgroup `tile`.
@@ -178,29 +176,52 @@ template <typename T> struct alignas(16) CxPair {
   T ar, ai, br, bi;
 };
 
-template <typename T, int M>
+// Global accesses of a tile address the array from the tile's first
+// element (a pointer that is the same for the whole workgroup) with 32-bit
+// byte offsets: the planner admits a pass to this path only when every
+// offset within a tile, on both sides, stays below 2^31 bytes.
+template <typename V> P3E_HD P3E_INL V ldg(const char *base, uint32_t off) {
+  return *reinterpret_cast<const V *>(base + off);
+}
+template <typename V> P3E_HD P3E_INL void stg(char *base, uint32_t off, const V &v) {
+  *reinterpret_cast<V *>(base + off) = v;
+}
+
+// FULL: the tile has all B lines (nb == B); every entry a thread owns is
+// then inside the tile and the accesses need no bounds checks.
+template <typename T, int M, bool FULL>
 P3E_HD P3E_INL void fast_fetch(const PassArgs &p, uint32_t f, int e0, int nb, TileRegs<T> &R, int tid, int nth) {
   constexpr int E = P3E_FAST_E;
-  const long long base = (long long)e0 * p.is_e + (long long)f * p.is_f;
+  constexpr int logM = Log2<M>::v;
+  constexpr uint32_t RS = sizeof(T), CS = 2 * sizeof(T);
+  const uint32_t esz = p.cls == CLS_R2C ? RS : CS;
+  const char *tin = reinterpret_cast<const char *>(p.in) + ((long long)e0 * p.is_e + (long long)f * p.is_f) * esz;
+  const int isd = (int)p.is_d, ise = (int)p.is_e;
   const int tot = (p.load_order == ORD_LINE ? nb : p.B) * p.nin;
   // out-of-tile entries load the tile's first element (always valid) and
   // are not written: unconditional loads keep the values in registers
-  if (p.vec2 && p.cls == CLS_C2C) {
-    // C2C, two adjacent elements per load: in tile order (e stride-1, whole
-    // tiles) element n of lines j = 2 jp and j + 1; in line order (d
-    // stride-1) elements n = 2 np and n + 1 of line j
-    const cx<T> *in = reinterpret_cast<const cx<T> *>(p.in);
-    const bool lo = p.load_order == ORD_LINE;
-    const int lb = lo ? Log2<M>::v - 1 : p.logB - 1;
-    const int ptot = lo ? nb * (M / 2) : (p.B >> 1) * p.nin;
+  if (p.vec2 && p.cls == CLS_C2C && p.load_order == ORD_LINE) {
+    // C2C, line order (d stride-1): elements n = 2 np and n + 1 of line j
 #pragma unroll
     for (int q = 0; q < E / 2; q++) {
       const int i = tid + q * nth;
-      const int lo_i = 2 * (i & ((1 << lb) - 1)), hi_i = i >> lb;
-      const int j = lo ? hi_i : lo_i, n = lo ? lo_i : hi_i;
-      const bool ok = i < ptot;
-      const CxPair<T> x = *reinterpret_cast<const CxPair<T> *>(
-          in + (ok ? base + (long long)n * p.is_d + (long long)j * p.is_e : base));
+      const int j = i >> (logM - 1), n = 2 * (i & (M / 2 - 1));
+      const bool ok = FULL || j < nb;
+      const CxPair<T> x = ldg<CxPair<T>>(tin, ok ? (uint32_t)(n + j * ise) * CS : 0u);
+      R.v[4 * q] = x.ar;
+      R.v[4 * q + 1] = x.ai;
+      R.v[4 * q + 2] = x.br;
+      R.v[4 * q + 3] = x.bi;
+    }
+  } else if (p.vec2 && p.cls == CLS_C2C) {
+    // C2C, tile order (e stride-1, whole tiles): element n of lines
+    // j = 2 jp and j + 1
+    const int lb = p.logB - 1;
+#pragma unroll
+    for (int q = 0; q < E / 2; q++) {
+      const int i = tid + q * nth;
+      const int j = 2 * (i & ((1 << lb) - 1)), n = i >> lb;
+      const CxPair<T> x = ldg<CxPair<T>>(tin, (uint32_t)(n * isd + j) * CS);
       R.v[4 * q] = x.ar;
       R.v[4 * q + 1] = x.ai;
       R.v[4 * q + 2] = x.br;
@@ -209,15 +230,12 @@ P3E_HD P3E_INL void fast_fetch(const PassArgs &p, uint32_t f, int e0, int nb, Ti
   } else if (p.vec2 && p.load_order == ORD_LINE) {
     // R2C, line order (d stride-1): entry q is the real pair (x_2m, x_2m+1)
     // of line j, the packed complex element m, read by one load
-    const T *in = reinterpret_cast<const T *>(p.in);
-    constexpr int logM = Log2<M>::v;
 #pragma unroll
     for (int q = 0; q < E; q++) {
       const int i = tid + q * nth;
       const int j = i >> logM, m = i & (M - 1);
-      const bool ok = j < nb;
-      const RealPair<T> x =
-          *reinterpret_cast<const RealPair<T> *>(in + (ok ? base + 2 * m + (long long)j * p.is_e : base));
+      const bool ok = FULL || j < nb;
+      const RealPair<T> x = ldg<RealPair<T>>(tin, ok ? (uint32_t)(2 * m + j * ise) * RS : 0u);
       R.v[2 * q] = x.a;
       R.v[2 * q + 1] = x.b;
     }
@@ -225,88 +243,91 @@ P3E_HD P3E_INL void fast_fetch(const PassArgs &p, uint32_t f, int e0, int nb, Ti
     // R2C, tile order, e stride-1 and whole tiles (the planner's condition):
     // entry q is element n of lines j = 2 jp and j + 1, adjacent in memory;
     // the launcher checked that base and strides keep the pair aligned
-    const T *in = reinterpret_cast<const T *>(p.in);
     const int lb = p.logB - 1;
 #pragma unroll
     for (int q = 0; q < E; q++) {
       const int i = tid + q * nth;
       const int jp = i & ((1 << lb) - 1), n = i >> lb;
-      const bool ok = i < (p.B >> 1) * p.nin;
-      const RealPair<T> x =
-          *reinterpret_cast<const RealPair<T> *>(in + (ok ? base + (long long)n * p.is_d + 2 * jp : base));
+      const RealPair<T> x = ldg<RealPair<T>>(tin, (uint32_t)(n * isd + 2 * jp) * RS);
       R.v[2 * q] = x.a;
       R.v[2 * q + 1] = x.b;
     }
   } else if (p.cls == CLS_R2C) {
-    const T *in = reinterpret_cast<const T *>(p.in);
 #pragma unroll
     for (int q = 0; q < 2 * E; q++) {
       const int i = tid + q * nth;
       int j, n;
       fast_load_index<M>(p, i, &j, &n);
-      const bool ok = i < tot && j < nb;
-      R.v[q] = in[ok ? base + (long long)n * p.is_d + (long long)j * p.is_e : base];
+      const bool ok = FULL ? i < tot : (i < tot && j < nb);
+      R.v[q] = ldg<T>(tin, ok ? (uint32_t)(n * isd + j * ise) * RS : 0u);
     }
   } else if (p.cls == CLS_C2R) {
-    // pairs (X_k, X_{M-k}) of one line in one thread, merged in fast_put
-    const cx<T> *in = reinterpret_cast<const cx<T> *>(p.in);
-    const int ptot = (p.load_order == ORD_LINE ? nb : p.B) * (M / 2 + 1);
+    // pairs (X_k, X_{M-k}) of one line in one thread, merged in fast_put;
+    // the first (E/2) nth pairs of (M/2 + 1) B are all inside the tile's
+    // pair range
 #pragma unroll
     for (int q = 0; q < E / 2; q++) {
       const int i = tid + q * nth;
       int j, k;
       fast_pair_index<M>(p, i, &j, &k);
-      const bool ok = i < ptot && j < nb;
-      const long long o = base + (long long)j * p.is_e;
-      const cx<T> a = in[ok ? o + (long long)k * p.is_d : base];
-      const cx<T> b = in[ok ? o + (long long)(M - k) * p.is_d : base];
+      const bool ok = FULL || j < nb;
+      const int o = j * ise;
+      const cx<T> a = ldg<cx<T>>(tin, ok ? (uint32_t)(o + k * isd) * CS : 0u);
+      const cx<T> b = ldg<cx<T>>(tin, ok ? (uint32_t)(o + (M - k) * isd) * CS : 0u);
       R.v[4 * q] = a.x;
       R.v[4 * q + 1] = a.y;
       R.v[4 * q + 2] = b.x;
       R.v[4 * q + 3] = b.y;
     }
   } else {
-    const cx<T> *in = reinterpret_cast<const cx<T> *>(p.in);
 #pragma unroll
     for (int q = 0; q < E; q++) {
       const int i = tid + q * nth;
       int j, n;
       fast_load_index<M>(p, i, &j, &n);
-      const bool ok = i < tot && j < nb;
-      const cx<T> x = in[ok ? base + (long long)n * p.is_d + (long long)j * p.is_e : base];
+      const bool ok = FULL ? i < tot : (i < tot && j < nb);
+      const cx<T> x = ldg<cx<T>>(tin, ok ? (uint32_t)(n * isd + j * ise) * CS : 0u);
       R.v[2 * q] = x.x;
       R.v[2 * q + 1] = x.y;
     }
   }
 }
 
-template <typename T, int M>
+template <typename T, int M, bool FULL>
 P3E_HD P3E_INL void fast_put(const PassArgs &p, uint32_t f, int e0, int nb, const TileRegs<T> &R, cx<T> *buf,
                              int tid, int nth) {
   constexpr int E = P3E_FAST_E;
+  constexpr int logM = Log2<M>::v;
   const int tot = (p.load_order == ORD_LINE ? nb : p.B) * p.nin;
-  if (p.vec2 && p.cls == CLS_C2C) {
-    const bool lo = p.load_order == ORD_LINE;
-    const int lb = lo ? Log2<M>::v - 1 : p.logB - 1;
-    const int ptot = lo ? nb * (M / 2) : (p.B >> 1) * p.nin;
+  const int lp = p.lp;
+  if (p.vec2 && p.cls == CLS_C2C && p.load_order == ORD_LINE) {
 #pragma unroll
     for (int q = 0; q < E / 2; q++) {
       const int i = tid + q * nth;
-      const int lo_i = 2 * (i & ((1 << lb) - 1)), hi_i = i >> lb;
-      const int j = lo ? hi_i : lo_i, n = lo ? lo_i : hi_i;
-      if (i < ptot) {
-        // the second element: line j + 1 (tile order) or element n + 1 (line order)
-        buf[j * p.lp + P3E_PAD(n)] = mk(R.v[4 * q], R.v[4 * q + 1]);
-        buf[lo ? j * p.lp + P3E_PAD(n + 1) : (j + 1) * p.lp + P3E_PAD(n)] = mk(R.v[4 * q + 2], R.v[4 * q + 3]);
+      const int j = i >> (logM - 1), n = 2 * (i & (M / 2 - 1));
+      if (FULL || j < nb) {
+        // n is even: n and n + 1 share a padding group
+        cx<T> *d = buf + j * lp + P3E_PAD(n);
+        d[0] = mk(R.v[4 * q], R.v[4 * q + 1]);
+        d[1] = mk(R.v[4 * q + 2], R.v[4 * q + 3]);
       }
     }
+  } else if (p.vec2 && p.cls == CLS_C2C) {
+    const int lb = p.logB - 1;
+#pragma unroll
+    for (int q = 0; q < E / 2; q++) {
+      const int i = tid + q * nth;
+      const int j = 2 * (i & ((1 << lb) - 1)), n = i >> lb;
+      cx<T> *d = buf + j * lp + P3E_PAD(n);
+      d[0] = mk(R.v[4 * q], R.v[4 * q + 1]);
+      d[lp] = mk(R.v[4 * q + 2], R.v[4 * q + 3]);
+    }
   } else if (p.vec2 && p.load_order == ORD_LINE) {
-    constexpr int logM = Log2<M>::v;
 #pragma unroll
     for (int q = 0; q < E; q++) {
       const int i = tid + q * nth;
       const int j = i >> logM, m = i & (M - 1);
-      if (j < nb) buf[j * p.lp + P3E_PAD(m)] = mk(R.v[2 * q], R.v[2 * q + 1]);
+      if (FULL || j < nb) buf[j * lp + P3E_PAD(m)] = mk(R.v[2 * q], R.v[2 * q + 1]);
     }
   } else if (p.vec2) {
     T *bufr = reinterpret_cast<T *>(buf);
@@ -315,11 +336,9 @@ P3E_HD P3E_INL void fast_put(const PassArgs &p, uint32_t f, int e0, int nb, cons
     for (int q = 0; q < E; q++) {
       const int i = tid + q * nth;
       const int j = 2 * (i & ((1 << lb) - 1)), n = i >> lb;
-      if (i < (p.B >> 1) * p.nin) {
-        const int o = 2 * P3E_PAD(n >> 1) + (n & 1);
-        bufr[2 * j * p.lp + o] = R.v[2 * q];
-        bufr[2 * (j + 1) * p.lp + o] = R.v[2 * q + 1];
-      }
+      const int o = 2 * P3E_PAD(n >> 1) + (n & 1);
+      bufr[2 * j * lp + o] = R.v[2 * q];
+      bufr[2 * (j + 1) * lp + o] = R.v[2 * q + 1];
     }
   } else if (p.cls == CLS_R2C) {
     T *bufr = reinterpret_cast<T *>(buf);
@@ -328,7 +347,7 @@ P3E_HD P3E_INL void fast_put(const PassArgs &p, uint32_t f, int e0, int nb, cons
       const int i = tid + q * nth;
       int j, n;
       fast_load_index<M>(p, i, &j, &n);
-      if (i < tot && j < nb) bufr[2 * (j * p.lp + P3E_PAD(n >> 1)) + (n & 1)] = R.v[q];
+      if (i < tot && (FULL || j < nb)) bufr[2 * (j * lp + P3E_PAD(n >> 1)) + (n & 1)] = R.v[q];
     }
   } else if (p.cls == CLS_C2R) {
     const int ptot = (p.load_order == ORD_LINE ? nb : p.B) * (M / 2 + 1);
@@ -337,18 +356,20 @@ P3E_HD P3E_INL void fast_put(const PassArgs &p, uint32_t f, int e0, int nb, cons
       const int i = tid + q * nth;
       int j, k;
       fast_pair_index<M>(p, i, &j, &k);
-      if (i < ptot && j < nb)
-        c2r_pair_put<T, M>(p, buf + j * p.lp, k, mk(R.v[4 * q], R.v[4 * q + 1]), mk(R.v[4 * q + 2], R.v[4 * q + 3]));
+      if (FULL || j < nb)
+        c2r_pair_put<T, M>(p, buf + j * lp, k, mk(R.v[4 * q], R.v[4 * q + 1]), mk(R.v[4 * q + 2], R.v[4 * q + 3]));
     }
     // (M/2 + 1) B pairs are B more than (E/2) nth: loaded here directly
-    const cx<T> *in = reinterpret_cast<const cx<T> *>(p.in);
-    const long long base = (long long)e0 * p.is_e + (long long)f * p.is_f;
+    const char *tin = reinterpret_cast<const char *>(p.in) +
+                      ((long long)e0 * p.is_e + (long long)f * p.is_f) * (long long)(2 * sizeof(T));
+    const int isd = (int)p.is_d, ise = (int)p.is_e;
     for (int i = tid + (E / 2) * nth; i < ptot; i += nth) {
       int j, k;
       fast_pair_index<M>(p, i, &j, &k);
       if (j >= nb) continue;
-      const long long o = base + (long long)j * p.is_e;
-      c2r_pair_put<T, M>(p, buf + j * p.lp, k, in[o + (long long)k * p.is_d], in[o + (long long)(M - k) * p.is_d]);
+      const int o = j * ise;
+      c2r_pair_put<T, M>(p, buf + j * lp, k, ldg<cx<T>>(tin, (uint32_t)(o + k * isd) * (uint32_t)(2 * sizeof(T))),
+                         ldg<cx<T>>(tin, (uint32_t)(o + (M - k) * isd) * (uint32_t)(2 * sizeof(T))));
     }
   } else {
 #pragma unroll
@@ -356,79 +377,127 @@ P3E_HD P3E_INL void fast_put(const PassArgs &p, uint32_t f, int e0, int nb, cons
       const int i = tid + q * nth;
       int j, n;
       fast_load_index<M>(p, i, &j, &n);
-      if (i < tot && j < nb) buf[j * p.lp + P3E_PAD(n)] = mk(R.v[2 * q], R.v[2 * q + 1]);
+      if (i < tot && (FULL || j < nb)) buf[j * lp + P3E_PAD(n)] = mk(R.v[2 * q], R.v[2 * q + 1]);
     }
   }
 }
 
-template <typename T, int M>
-P3E_HD void fast_load(const PassArgs &p, uint32_t f, int e0, int nb, cx<T> *buf, int tid, int nth) {
+template <typename T, int M, bool FULL>
+P3E_HD P3E_INL void fast_load(const PassArgs &p, uint32_t f, int e0, int nb, cx<T> *buf, int tid, int nth) {
   TileRegs<T> R;
-  fast_fetch<T, M>(p, f, e0, nb, R, tid, nth);
-  fast_put<T, M>(p, f, e0, nb, R, buf, tid, nth);
+  fast_fetch<T, M, FULL>(p, f, e0, nb, R, tid, nth);
+  fast_put<T, M, FULL>(p, f, e0, nb, R, buf, tid, nth);
+}
+
+// Output rule of element k of line j (line j + e0 of the array): the
+// derivative i k, the Poisson operator, the scale.
+template <typename T>
+P3E_HD P3E_INL cx<T> out_rule(const PassArgs &p, cx<T> y, int k, int je, uint32_t f) {
+  if (p.deriv) {
+    T kk;
+    if (k < p.dmid) kk = (T)k;
+    else if (k == p.dmid) kk = (T)0;
+    else kk = (T)(k - p.dg);
+    y = mk(-kk * y.y, kk * y.x);
+  }
+  if (p.spec_op) y = scl(y, poisson_factor<T>(p, k, je, (int)f));
+  if (p.has_scale) y = scl(y, (T)p.scale);
+  return y;
+}
+
+// R2C output k of a line from the packed spectrum Z:
+//   X_k = (Z_k + conj Z_{M-k})/2 - i w_N^k (Z_k - conj Z_{M-k})/2
+template <typename T, int M>
+P3E_HD P3E_INL cx<T> r2c_split(const cx<T> *l, const cx<T> *wN, int k) {
+  const cx<T> zk = l[P3E_PAD(k & (M - 1))];
+  const cx<T> zm = l[P3E_PAD((M - k) & (M - 1))];
+  const cx<T> e = add(zk, conj(zm)), o = sub(zk, conj(zm));
+  return scl(add(e, rot(mul(wN[k], o), -1)), (T)0.5);
+}
+
+// Tile-order paired store of entry i (e stride-1, whole tiles): element k
+// of lines j = 2 jp and j + 1, adjacent in memory, by one store.
+template <typename T, int M>
+P3E_HD P3E_INL void store_pair_tile(const PassArgs &p, uint32_t f, int e0, const cx<T> *buf, const cx<T> *wN,
+                                    char *tout, int i) {
+  const int lb = p.logB - 1, lp = p.lp;
+  const int j = 2 * (i & ((1 << lb) - 1)), k = i >> lb;
+  cx<T> y0, y1;
+  if (p.cls == CLS_R2C) {
+    y0 = r2c_split<T, M>(buf + j * lp, wN, k);
+    y1 = r2c_split<T, M>(buf + (j + 1) * lp, wN, k);
+  } else {
+    y0 = buf[j * lp + P3E_PAD(k)];
+    y1 = buf[(j + 1) * lp + P3E_PAD(k)];
+  }
+  y0 = out_rule(p, y0, k, e0 + j, f);
+  y1 = out_rule(p, y1, k, e0 + j + 1, f);
+  CxPair<T> v;
+  v.ar = y0.x;
+  v.ai = y0.y;
+  v.br = y1.x;
+  v.bi = y1.y;
+  stg(tout, (uint32_t)(k * (int)p.os_d + j) * (uint32_t)(2 * sizeof(T)), v);
 }
 
 // Store phase: output rule and LDS -> tile.
-template <typename T, int M>
-P3E_HD void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, const cx<T> *buf, int tid, int nth) {
+template <typename T, int M, bool FULL>
+P3E_HD P3E_INL void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, const cx<T> *buf, int tid, int nth) {
   constexpr int logM = Log2<M>::v;
-  const cx<T> *wN = reinterpret_cast<const cx<T> *>(p.wN);
-  const long long base = (long long)e0 * p.os_e + (long long)f * p.os_f;
-  const int tot = (p.store_order == ORD_LINE ? nb : p.B) * p.nout;
   constexpr int E = P3E_FAST_E;
+  constexpr uint32_t RS = sizeof(T), CS = 2 * sizeof(T);
+  const cx<T> *wN = reinterpret_cast<const cx<T> *>(p.wN);
+  const uint32_t esz = p.cls == CLS_C2R ? RS : CS;
+  char *tout = reinterpret_cast<char *>(p.out) + ((long long)e0 * p.os_e + (long long)f * p.os_f) * esz;
+  const int osd = (int)p.os_d, ose = (int)p.os_e;
+  const int lp = p.lp;
+  const int tot = (p.store_order == ORD_LINE ? nb : p.B) * p.nout;
   if (p.vec2s && p.cls == CLS_C2R && p.store_order == ORD_LINE) {
     // C2R, line order (d stride-1): the packed complex element m of line j
     // is the real pair (x_2m, x_2m+1), written by one store
-    T *out = reinterpret_cast<T *>(p.out);
 #pragma unroll 4
-    for (int i = tid; i < nb * M; i += nth) {
+    for (int q = 0; q < E; q++) {
+      const int i = tid + q * nth;
       const int j = i >> logM, m = i & (M - 1);
-      const cx<T> z = buf[j * p.lp + P3E_PAD(m)];
+      if (!FULL && j >= nb) continue;
+      const cx<T> z = buf[j * lp + P3E_PAD(m)];
       RealPair<T> v;
       v.a = p.has_scale ? z.x * (T)p.scale : z.x;
       v.b = p.has_scale ? z.y * (T)p.scale : z.y;
-      *reinterpret_cast<RealPair<T> *>(out + base + 2 * m + (long long)j * p.os_e) = v;
+      stg(tout, (uint32_t)(2 * m + j * ose) * RS, v);
     }
   } else if (p.vec2s && p.cls == CLS_C2R) {
     // C2R, tile order, e stride-1 and whole tiles (the planner's condition):
     // entry q is element k of lines j = 2 jp and j + 1, adjacent in memory,
     // written by one store; the launcher checked the pair alignment
-    T *out = reinterpret_cast<T *>(p.out);
     const T *bufr = reinterpret_cast<const T *>(buf);
     const int lb = p.logB - 1;
     T y[2 * E];
-    long long off[E];
+    uint32_t off[E];
 #pragma unroll
     for (int q = 0; q < E; q++) {
       const int i = tid + q * nth;
       const int j = 2 * (i & ((1 << lb) - 1)), k = i >> lb;
-      const bool ok = i < (p.B >> 1) * p.nout;
       const int o = 2 * P3E_PAD(k >> 1) + (k & 1);
-      y[2 * q] = bufr[ok ? 2 * j * p.lp + o : 0];
-      y[2 * q + 1] = bufr[ok ? 2 * (j + 1) * p.lp + o : 0];
-      off[q] = ok ? base + (long long)k * p.os_d + j : -1;
+      y[2 * q] = bufr[2 * j * lp + o];
+      y[2 * q + 1] = bufr[2 * (j + 1) * lp + o];
+      off[q] = (uint32_t)(k * osd + j) * RS;
     }
 #pragma unroll
     for (int q = 0; q < E; q++) {
-      if (off[q] < 0) continue;
       RealPair<T> v;
       v.a = p.has_scale ? y[2 * q] * (T)p.scale : y[2 * q];
       v.b = p.has_scale ? y[2 * q + 1] * (T)p.scale : y[2 * q + 1];
-      *reinterpret_cast<RealPair<T> *>(out + off[q]) = v;
+      stg(tout, off[q], v);
     }
   } else if (p.cls == CLS_C2R) {
-    T *out = reinterpret_cast<T *>(p.out);
     const T *bufr = reinterpret_cast<const T *>(buf);
-    // 2E reals per thread, 2CH at a time: LDS reads first, then the global
-    // stores
-    constexpr int CH = E;
+    // 2E reals per thread: LDS reads first, then the global stores
+    T y[2 * E];
+    int off[2 * E];
 #pragma unroll
-    for (int c = 0; c < 2 * E; c += 2 * CH) {
-    T y[2 * CH];
-    long long off[2 * CH];
-#pragma unroll
-    for (int q = 0; q < 2 * CH; q++) {
-      const int i = tid + (c + q) * nth;
+    for (int q = 0; q < 2 * E; q++) {
+      const int i = tid + q * nth;
       int k, j;
       if (p.store_order == ORD_LINE) {
         j = i >> (logM + 1);
@@ -437,64 +506,38 @@ P3E_HD void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, const cx<T
         j = i & (p.B - 1);
         k = i >> p.logB;
       }
-      const bool ok = i < tot && j < nb;
-      y[q] = bufr[ok ? 2 * (j * p.lp + P3E_PAD(k >> 1)) + (k & 1) : 0];
-      off[q] = ok ? base + (long long)k * p.os_d + (long long)j * p.os_e : -1;
+      const bool ok = i < tot && (FULL || j < nb);
+      y[q] = bufr[ok ? 2 * (j * lp + P3E_PAD(k >> 1)) + (k & 1) : 0];
+      off[q] = ok ? k * osd + j * ose : -1;
     }
 #pragma unroll
-    for (int q = 0; q < 2 * CH; q++)
-      if (off[q] >= 0) out[off[q]] = p.has_scale ? y[q] * (T)p.scale : y[q];
+    for (int q = 0; q < 2 * E; q++)
+      if (off[q] >= 0) stg(tout, (uint32_t)off[q] * RS, p.has_scale ? y[q] * (T)p.scale : y[q]);
+  } else if (p.vec2s && p.store_order == ORD_LINE) {
+    // C2C, line order (d stride-1): elements k = 2 kp and k + 1 of line j
+#pragma unroll
+    for (int q = 0; q < E / 2; q++) {
+      const int i = tid + q * nth;
+      const int j = i >> (logM - 1), k = 2 * (i & (M / 2 - 1));
+      if (!FULL && j >= nb) continue;
+      const cx<T> *l = buf + j * lp + P3E_PAD(k);
+      const cx<T> y0 = out_rule(p, l[0], k, e0 + j, f), y1 = out_rule(p, l[1], k + 1, e0 + j, f);
+      CxPair<T> v;
+      v.ar = y0.x;
+      v.ai = y0.y;
+      v.br = y1.x;
+      v.bi = y1.y;
+      stg(tout, (uint32_t)(k + j * ose) * CS, v);
     }
   } else if (p.vec2s) {
-    // C2C and R2C, two adjacent output elements per store: in tile order (e
-    // stride-1, whole tiles) element k of lines j = 2 jp and j + 1; in line
-    // order (C2C only, d stride-1) elements k = 2 kp and k + 1 of line j
-    cx<T> *out = reinterpret_cast<cx<T> *>(p.out);
-    const bool lo = p.store_order == ORD_LINE;
-    const int lb = lo ? Log2<M>::v - 1 : p.logB - 1;
-    const int ptot = lo ? nb * (M / 2) : (p.B >> 1) * p.nout;
-#pragma unroll 4
-    for (int i = tid; i < ptot; i += nth) {
-      const int lo_i = 2 * (i & ((1 << lb) - 1)), hi_i = i >> lb;
-      const int j = lo ? hi_i : lo_i, k = lo ? lo_i : hi_i;
-      const int j1 = lo ? j : j + 1, k1 = lo ? k + 1 : k;  // the second element
-      cx<T> y[2];
-      if (p.cls == CLS_R2C) {
-        // X_k = (Z_k + conj Z_{M-k})/2 - i w_N^k (Z_k - conj Z_{M-k})/2, per line
+    // C2C and R2C, tile order (e stride-1, whole tiles): element k of lines
+    // j = 2 jp and j + 1, one store.  (E/2) nth entries are exactly the C2C
+    // tile (nout = M); an R2C tile has nout = M + 1, B/2 entries more.
+    const int ptot = (p.B >> 1) * p.nout;
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
-          const cx<T> *l = buf + (h ? j1 : j) * p.lp;
-          const cx<T> zk = l[P3E_PAD(k & (M - 1))];
-          const cx<T> zm = l[P3E_PAD((M - k) & (M - 1))];
-          const cx<T> e = add(zk, conj(zm)), o = sub(zk, conj(zm));
-          y[h] = scl(add(e, rot(mul(wN[k], o), -1)), (T)0.5);
-        }
-      } else {
-        y[0] = buf[j * p.lp + P3E_PAD(k)];
-        y[1] = buf[j1 * p.lp + P3E_PAD(k1)];
-      }
-#pragma unroll
-      for (int h = 0; h < 2; h++) {
-        const int kh = h ? k1 : k, jh = h ? j1 : j;
-        if (p.deriv) {
-          T kk;
-          if (kh < p.dmid) kk = (T)kh;
-          else if (kh == p.dmid) kk = (T)0;
-          else kk = (T)(kh - p.dg);
-          y[h] = mk(-kk * y[h].y, kk * y[h].x);
-        }
-        if (p.spec_op) y[h] = scl(y[h], poisson_factor<T>(p, kh, e0 + jh, (int)f));
-        if (p.has_scale) y[h] = scl(y[h], (T)p.scale);
-      }
-      CxPair<T> v;
-      v.ar = y[0].x;
-      v.ai = y[0].y;
-      v.br = y[1].x;
-      v.bi = y[1].y;
-      *reinterpret_cast<CxPair<T> *>(out + base + (long long)k * p.os_d + (long long)j * p.os_e) = v;
-    }
+    for (int q = 0; q < E / 2; q++) store_pair_tile<T, M>(p, f, e0, buf, wN, tout, tid + q * nth);
+    for (int i = tid + (E / 2) * nth; i < ptot; i += nth) store_pair_tile<T, M>(p, f, e0, buf, wN, tout, i);
   } else {
-    cx<T> *out = reinterpret_cast<cx<T> *>(p.out);
 #pragma unroll 4
     for (int i = tid; i < tot; i += nth) {
       int k, j;
@@ -506,32 +549,15 @@ P3E_HD void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, const cx<T
           j = i >> logM;
           k = i & (M - 1);
         }
+        if (!FULL && j >= nb) continue;
       } else {
         j = i & (p.B - 1);
         k = i >> p.logB;
-        if (j >= nb) continue;
+        if (!FULL && j >= nb) continue;
       }
-      const cx<T> *l = buf + j * p.lp;
-      cx<T> y;  // output element k of line j
-      if (p.cls == CLS_R2C) {
-        // X_k = (Z_k + conj Z_{M-k})/2 - i w_N^k (Z_k - conj Z_{M-k})/2
-        const cx<T> zk = l[P3E_PAD(k & (M - 1))];
-        const cx<T> zm = l[P3E_PAD((M - k) & (M - 1))];
-        const cx<T> e = add(zk, conj(zm)), o = sub(zk, conj(zm));
-        y = scl(add(e, rot(mul(wN[k], o), -1)), (T)0.5);
-      } else {
-        y = l[P3E_PAD(k)];
-      }
-      if (p.deriv) {
-        T kk;
-        if (k < p.dmid) kk = (T)k;
-        else if (k == p.dmid) kk = (T)0;
-        else kk = (T)(k - p.dg);
-        y = mk(-kk * y.y, kk * y.x);
-      }
-      if (p.spec_op) y = scl(y, poisson_factor<T>(p, k, e0 + j, (int)f));
-      if (p.has_scale) y = scl(y, (T)p.scale);
-      out[base + (long long)k * p.os_d + (long long)j * p.os_e] = y;
+      const cx<T> *l = buf + j * lp;
+      const cx<T> y = p.cls == CLS_R2C ? r2c_split<T, M>(l, wN, k) : l[P3E_PAD(k)];
+      stg(tout, (uint32_t)(k * osd + j * ose) * CS, out_rule(p, y, k, e0 + j, f));
     }
   }
 }
@@ -540,18 +566,26 @@ P3E_HD void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, const cx<T
 // barrier (device), or runs it for every thread of the workgroup in turn
 // (host build); `run` does the same for the two halves of a radix pass.
 // Only threads tid < nth work (a batch workgroup may be larger than the
-// pass needs); the others take part in the barriers.
+// pass needs); the others take part in the barriers.  The branches on the
+// tile being whole and on the DFT sign are uniform over the workgroup.
 template <typename T, int M, typename Run, typename Each>
-P3E_HD void fast_tile(const PassArgs &p, uint32_t tile, cx<T> *buf, int nth, Run &run, Each &each) {
+P3E_HD P3E_INL void fast_tile(const PassArgs &p, uint32_t tile, cx<T> *buf, int nth, Run &run, Each &each) {
   uint32_t f;
   int e0, nb;
   fast_tile_coords(p, tile, &f, &e0, &nb);
-  each([&](int tid) {
-    if (tid < nth) fast_load<T, M>(p, f, e0, nb, buf, tid, nth);
+  const bool full = nb == p.B;
+  each([&](int tid) P3E_LAMBDA_INL {
+    if (tid >= nth) return;
+    if (full) fast_load<T, M, true>(p, f, e0, nb, buf, tid, nth);
+    else fast_load<T, M, false>(p, f, e0, nb, buf, tid, nth);
   });
-  fft_pow2<T, M>(run, buf, p.lp, nb, (p.cls == CLS_R2C) ? -1 : p.sign, reinterpret_cast<const cx<T> *>(p.tw));
-  each([&](int tid) {
-    if (tid < nth) fast_store<T, M>(p, f, e0, nb, buf, tid, nth);
+  const cx<T> *tw = reinterpret_cast<const cx<T> *>(p.tw);
+  if (p.cls == CLS_R2C || p.sign < 0) fft_pow2<T, M, -1>(run, buf, p.lp, tw);
+  else fft_pow2<T, M, 1>(run, buf, p.lp, tw);
+  each([&](int tid) P3E_LAMBDA_INL {
+    if (tid >= nth) return;
+    if (full) fast_store<T, M, true>(p, f, e0, nb, buf, tid, nth);
+    else fast_store<T, M, false>(p, f, e0, nb, buf, tid, nth);
   });
 }
 

@@ -60,7 +60,7 @@ __global__ void __launch_bounds__(256) pass_kernel(PassArgs p) {
 // barrier, write half, barrier; the thread's values live in registers.
 template <typename T> struct DevPassRun {
   int tid, nth;
-  template <class PS> __device__ void operator()(PS &ps) const {
+  template <class PS> __device__ __forceinline__ void operator()(PS &ps) const {
     cx<T> v[PS::SLOTS];
     if (tid < nth) ps.read(tid, nth, v);
     __syncthreads();
@@ -72,7 +72,7 @@ template <typename T> struct DevPassRun {
 // A plain phase of a tile for this thread, then a barrier.
 struct DevEach {
   int tid;
-  template <class F> __device__ void operator()(F f) const {
+  template <class F> __device__ __forceinline__ void operator()(F f) const {
     f(tid);
     __syncthreads();
   }
