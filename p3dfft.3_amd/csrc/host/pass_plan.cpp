@@ -175,6 +175,7 @@ static int generic_pitch(int L) {
 static const size_t LDS_BUDGET = 80 * 1024;
 static const size_t LDS_MAX = 160 * 1024;
 
+
 static bool g_fast_enabled = true;
 void set_fast_path(bool enable) { g_fast_enabled = enable; }
 
@@ -291,6 +292,9 @@ static void setup_fast(int prec, const OpSpec &op, PassDesc &ps, int batch_threa
   // after the tile; the batch kernel reads them from global memory
   const size_t tables = batch_threads > 0 ? 0 : (size_t)(M + (op.cls == CLS_C2C ? 0 : M + 1)) * esz;
   auto lds = [&](int b) { return (size_t)b * lp * esz + tables; };
+  // (a 160 KiB budget, B = 64 lines of 256 or 32 of 512 with one workgroup
+  // per CU, measured slower on MI355X: 0.31 ms vs 0.27 ms for the R2C pass
+  // of a 512^3 grid, 0.30 ms vs 0.23 ms for a C2C pass)
   while (B * 2 <= Bmax && lds(B * 2) <= LDS_BUDGET && (long)M * B * 2 / P3E_FAST_E <= maxth) B *= 2;
   if (lds(B) > LDS_MAX || (long)M * B / P3E_FAST_E > maxth) return;
   // the kernel addresses a tile's elements with 32-bit byte offsets from

@@ -21,7 +21,10 @@
 
 namespace p3e {
 
-template <typename T> struct cx {
+// Aligned to its size, so that one complex value moves with one 8-byte
+// (FP32) or 16-byte (FP64) LDS or memory instruction, and two values at
+// compile-time offsets with one paired LDS instruction.
+template <typename T> struct alignas(2 * sizeof(T)) cx {
   T x, y;
 };
 

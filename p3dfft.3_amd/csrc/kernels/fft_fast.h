@@ -40,14 +40,24 @@ template <typename T, int R, int M, int Ns, int S> struct FPass {
   int lp;
   const cx<T> *tw;
 
+  // Padded LDS positions as a per-thread base plus a compile-time offset per
+  // r (so the compiler folds the offsets into the LDS instructions):
+  // P3E_PAD(x + r STEP) - P3E_PAD(x) = pad_off<STEP>(r) when STEP is a
+  // multiple of 16 (any x), or when 16 is a multiple of STEP and x is below
+  // STEP (x + r STEP then never crosses more 16-element groups than r STEP).
+  template <int STEP> static constexpr int pad_off(int r) {
+    return STEP % 16 == 0 ? r * (STEP + STEP / 16) : r * STEP + ((r * STEP) >> 4);
+  }
+
   P3E_HD P3E_INL void read(int tid, int nth, cx<T> *v) const {
 #pragma unroll
     for (int m = 0; m < NB; m++) {
       const int b = tid + m * nth;
-      const cx<T> *sl = buf + (b >> lognbf) * lp;
       const int j = b & (nbf - 1);
+      // elements j + r nbf, j < nbf
+      const cx<T> *sl = buf + (b >> lognbf) * lp + (nbf % 16 == 0 ? P3E_PAD(j) : j);
 #pragma unroll
-      for (int r = 0; r < R; r++) v[m * R + r] = sl[P3E_PAD(j + r * nbf)];
+      for (int r = 0; r < R; r++) v[m * R + r] = sl[pad_off<nbf>(r)];
     }
   }
 
@@ -64,9 +74,16 @@ template <typename T, int R, int M, int Ns, int S> struct FPass {
       }
       bfly<T, R>(w, S);
       cx<T> *dl = buf + (b >> lognbf) * lp;
-      const int base = ((j - k) << logR) + k;
+      // outputs hi + k + r Ns, hi = (j - k) R a multiple of Ns R, k < Ns
+      const int hi = (j - k) << logR;
+      if constexpr ((Ns * R) % 16 == 0) {
+        cx<T> *d = dl + (Ns % 16 == 0 ? P3E_PAD(hi + k) : P3E_PAD(hi) + k);
 #pragma unroll
-      for (int r = 0; r < R; r++) dl[P3E_PAD(base + r * Ns)] = w[r];
+        for (int r = 0; r < R; r++) d[pad_off<Ns>(r)] = w[r];
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; r++) dl[P3E_PAD(hi + k + r * Ns)] = w[r];
+      }
     }
   }
 };

@@ -20,23 +20,9 @@
 #include "fft_core.h"
 #include "fft_fast.h"
 #include "fft_mid.h"
+#include "pass_dev.h"
 
 namespace p3e {
-
-struct DevBarrier {
-  __device__ void operator()() const { __syncthreads(); }
-};
-
-// Workgroups are dispatched to the 8 XCDs of an MI355X round-robin (block b
-// to XCD b % 8), and each XCD has its own L2.  Tiles that are adjacent along
-// the tile axis share cache lines when their rows are shorter than a line,
-// so consecutive tiles are given to workgroups of the same XCD: block b
-// takes tile (b % 8) * (G / 8) + b / 8 (the last G % 8 blocks keep their
-// index).  A bijection on [0, G).
-__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t G) {
-  const uint32_t per = G >> 3;
-  return b < (per << 3) ? (b & 7u) * per + (b >> 3) : b;
-}
 
 // Generic path: two LDS line buffers, then the L twiddles (copied from
 // global memory before the first barrier of the tile).
@@ -54,81 +40,6 @@ __global__ void __launch_bounds__(256) pass_kernel(PassArgs p) {
   // one call site: a second inlined copy of the tile code makes the
   // compiler keep PassArgs in scratch memory
   run_tile<T>(p, xcd_tile(blockIdx.x, gridDim.x), bufA, bufB, threadIdx.x, blockDim.x, DevBarrier(), tw);
-}
-
-// One radix pass of the power-of-two path for this thread: read half,
-// barrier, write half, barrier; the thread's values live in registers.
-template <typename T> struct DevPassRun {
-  int tid, nth;
-  template <class PS> __device__ __forceinline__ void operator()(PS &ps) const {
-    cx<T> v[PS::SLOTS];
-    if (tid < nth) ps.read(tid, nth, v);
-    __syncthreads();
-    if (tid < nth) ps.write(tid, nth, v);
-    __syncthreads();
-  }
-};
-
-// A plain phase of a tile for this thread, then a barrier.
-struct DevEach {
-  int tid;
-  template <class F> __device__ __forceinline__ void operator()(F f) const {
-    f(tid);
-    __syncthreads();
-  }
-};
-
-// Power-of-two path: complex length M at compile time, one LDS buffer for
-// the tile, followed in LDS by the twiddle table (M entries) and, for
-// R2C/C2R, the split factors (M + 1 entries): the radix passes and the
-// split/merge read them from LDS instead of issuing global loads (which
-// would compete with the tile's stream for the memory pipeline).  The
-// twiddles are first read after the barrier that ends the load phase; the
-// split factors are read by the C2R load phase itself, so R2C/C2R passes
-// wait at a barrier after copying them.
-template <typename T, int M>
-__global__ void __launch_bounds__(P3E_FAST_MAX_THREADS) pass_kernel_pow2(PassArgs p0) {
-  extern __shared__ unsigned char smem_raw[];
-  cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
-  cx<T> *ltw = buf + (size_t)p0.B * p0.lp;
-  cx<T> *lwn = ltw + M;
-  const cx<T> *gtw = reinterpret_cast<const cx<T> *>(p0.tw);
-  const cx<T> *gwn = reinterpret_cast<const cx<T> *>(p0.wN);
-  const int tid = threadIdx.x, nth = blockDim.x;
-  for (int i = tid; i < p0.ntw; i += nth) ltw[i] = gtw[i];
-  if (p0.cls != CLS_C2C) {
-    for (int i = tid; i <= M; i += nth) lwn[i] = gwn[i];
-    // the C2R load phase merges (X_k, X_{M-k}) with the split factors, so
-    // they must be in LDS before any thread's load phase
-    __syncthreads();
-  }
-  PassArgs p = p0;
-  p.tw = ltw;
-  p.wN = lwn;
-  DevPassRun<T> run{tid, nth};
-  DevEach each{tid};
-  fast_tile<T, M>(p, blockIdx.x, buf, nth, run, each);
-}
-
-// Single-buffer mixed-radix path: one padded LDS buffer, then the twiddles
-// when they fit.  P: the odd prime radix of the pass (0: none).
-template <typename T, int P>
-__global__ void __launch_bounds__(P3E_MID_THREADS(sizeof(T), P)) pass_kernel_mid(PassArgs p) {
-  extern __shared__ unsigned char smem_raw[];
-  cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
-  const cx<T> *tw = reinterpret_cast<const cx<T> *>(p.tw);
-  const int tid = threadIdx.x, nth = blockDim.x;
-  if (p.tw_in_lds) {
-    cx<T> *ltw = buf + (size_t)tile_lines(p, p.B) * p.lpad;
-    for (int i = tid; i < p.ntw; i += nth) ltw[i] = tw[i];
-    tw = ltw;
-  }
-  DevPassRun<T> run{tid, nth};
-  DevEach each{tid};
-  // one call site (the twiddle reads then compile to flat loads, as tw may
-  // point to LDS or to global memory): two inlined copies, each with one
-  // address space, measured slower on MI355X
-  mid_tile<T, P>(p, xcd_tile(blockIdx.x, gridDim.x), buf, nth, run, each, tw);
 }
 
 // Batch step: a grid-stride loop over a work list of (pass, tile) entries
@@ -184,65 +95,6 @@ __global__ void __launch_bounds__(P3E_STEP_THREADS)
   }
 }
 
-template <typename T, int M>
-static int launch_pow2(const PassArgs *a, size_t lds_bytes, int threads, unsigned int nblocks, hipStream_t st) {
-  if (lds_bytes > 65536)
-    (void)hipFuncSetAttribute((const void *)pass_kernel_pow2<T, M>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lds_bytes);
-  hipLaunchKernelGGL((pass_kernel_pow2<T, M>), dim3(nblocks), dim3(threads), lds_bytes, st, *a);
-  return (int)hipGetLastError();
-}
-
-template <typename T>
-static int launch_pow2_any(const PassArgs *a, size_t lds_bytes, int threads, unsigned int nblocks, hipStream_t st) {
-  switch (a->M) {
-    case 16: return launch_pow2<T, 16>(a, lds_bytes, threads, nblocks, st);
-    case 32: return launch_pow2<T, 32>(a, lds_bytes, threads, nblocks, st);
-    case 64: return launch_pow2<T, 64>(a, lds_bytes, threads, nblocks, st);
-    case 128: return launch_pow2<T, 128>(a, lds_bytes, threads, nblocks, st);
-    case 256: return launch_pow2<T, 256>(a, lds_bytes, threads, nblocks, st);
-    case 512: return launch_pow2<T, 512>(a, lds_bytes, threads, nblocks, st);
-    case 1024: return launch_pow2<T, 1024>(a, lds_bytes, threads, nblocks, st);
-    case 2048: return launch_pow2<T, 2048>(a, lds_bytes, threads, nblocks, st);
-    case 4096: return launch_pow2<T, 4096>(a, lds_bytes, threads, nblocks, st);
-  }
-  return (int)hipErrorInvalidValue;
-}
-
-template <typename T, int P>
-static int launch_mid(const PassArgs *a, size_t lds_bytes, int threads, unsigned int nblocks, hipStream_t st) {
-  if (lds_bytes > 65536)
-    (void)hipFuncSetAttribute((const void *)pass_kernel_mid<T, P>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lds_bytes);
-  hipLaunchKernelGGL((pass_kernel_mid<T, P>), dim3(nblocks), dim3(threads), lds_bytes, st, *a);
-  return (int)hipGetLastError();
-}
-
-template <typename T>
-static int launch_mid_any(const PassArgs *a, size_t lds_bytes, int threads, unsigned int nblocks, hipStream_t st) {
-  switch (a->mid_prime) {
-    case 0: return launch_mid<T, 0>(a, lds_bytes, threads, nblocks, st);
-    case 7: return launch_mid<T, 7>(a, lds_bytes, threads, nblocks, st);
-    case 11: return launch_mid<T, 11>(a, lds_bytes, threads, nblocks, st);
-    case 13: return launch_mid<T, 13>(a, lds_bytes, threads, nblocks, st);
-    case 17: return launch_mid<T, 17>(a, lds_bytes, threads, nblocks, st);
-    case 19: return launch_mid<T, 19>(a, lds_bytes, threads, nblocks, st);
-    case 23: return launch_mid<T, 23>(a, lds_bytes, threads, nblocks, st);
-    case 29: return launch_mid<T, 29>(a, lds_bytes, threads, nblocks, st);
-  }
-  // primes above 29: FP32 only (the planner's limit for FP64)
-  if constexpr (sizeof(T) == 4) {
-    switch (a->mid_prime) {
-      case 31: return launch_mid<T, 31>(a, lds_bytes, threads, nblocks, st);
-      case 37: return launch_mid<T, 37>(a, lds_bytes, threads, nblocks, st);
-      case 41: return launch_mid<T, 41>(a, lds_bytes, threads, nblocks, st);
-      case 43: return launch_mid<T, 43>(a, lds_bytes, threads, nblocks, st);
-      case 47: return launch_mid<T, 47>(a, lds_bytes, threads, nblocks, st);
-    }
-  }
-  return (int)hipErrorInvalidValue;
-}
-
 // ---------------------------------------------------------------- compute_deriv
 // build/deriv.C:85-186: multiply a stored spectrum by i*k along storage rank
 // ldir, with wavenumbers from the logical axis (g, mid, glob_start).
@@ -263,6 +115,11 @@ __global__ void deriv_kernel(const cx<T> *in, cx<T> *out, long long n, int s0, i
   out[p] = mk(-kk * v.y, kk * v.x);
 }
 
+int launch_pow2_pass(const PassArgs *a, int prec, size_t lds_bytes, int threads, unsigned int nblocks,
+                     hipStream_t st);  // fft_pow2.hip
+int launch_mid_pass(const PassArgs *a, int prec, size_t lds_bytes, int threads, unsigned int nblocks,
+                    hipStream_t st);  // fft_midk.hip
+
 }  // namespace p3e
 
 // ---------------------------------------------------------------- launchers
@@ -281,16 +138,13 @@ extern "C" int p3e_launch_pass(const p3e::PassArgs *a, int prec, size_t lds_byte
       p3e::PassArgs b = *a;
       if (in_odd) b.vec2 = 0;
       if (out_odd) b.vec2s = 0;
-      return prec == 8 ? p3e::launch_pow2_any<double>(&b, lds_bytes, threads, nblocks, st)
-                       : p3e::launch_pow2_any<float>(&b, lds_bytes, threads, nblocks, st);
+      return p3e::launch_pow2_pass(&b, prec, lds_bytes, threads, nblocks, st);
     }
-    return prec == 8 ? p3e::launch_pow2_any<double>(a, lds_bytes, threads, nblocks, st)
-                     : p3e::launch_pow2_any<float>(a, lds_bytes, threads, nblocks, st);
+    return p3e::launch_pow2_pass(a, prec, lds_bytes, threads, nblocks, st);
   }
   if (a->mid) {
     if (threads < 1 || threads > P3E_MID_THREADS(prec, a->mid_prime)) return (int)hipErrorInvalidValue;
-    return prec == 8 ? p3e::launch_mid_any<double>(a, lds_bytes, threads, nblocks, st)
-                     : p3e::launch_mid_any<float>(a, lds_bytes, threads, nblocks, st);
+    return p3e::launch_mid_pass(a, prec, lds_bytes, threads, nblocks, st);
   }
   if (prec == 8) {
     if (lds_bytes > 65536)
