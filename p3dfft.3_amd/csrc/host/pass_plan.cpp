@@ -31,23 +31,47 @@ void clear_error() {
 }
 
 // ------------------------------------------------------------------ helpers
-size_t layout_elems(const Layout &l) { return (size_t)l.dims[0] * l.dims[1] * l.dims[2]; }
+static int stride1_axis(const int mo[3]) {
+  for (int a = 0; a < 3; a++)
+    if (mo[a] == 0) return a;
+  return 0;
+}
+
+// Allocated extent of logical axis a (the pitch for the stride-1 axis).
+static int extent(const Layout &l, int a) { return (l.mo[a] == 0 && l.pitch > l.dims[a]) ? l.pitch : l.dims[a]; }
+
+size_t layout_elems(const Layout &l) { return (size_t)extent(l, 0) * extent(l, 1) * extent(l, 2); }
 size_t elem_bytes(int prec, int dt) { return (size_t)prec * dt; }
 
 static void strides_of(const Layout &l, long long st[3]) {
   for (int a = 0; a < 3; a++) {
     long long s = 1;
     for (int b = 0; b < 3; b++)
-      if (l.mo[b] < l.mo[a]) s *= l.dims[b];
+      if (l.mo[b] < l.mo[a]) s *= extent(l, b);
     st[a] = s;
   }
 }
 
-static int stride1_axis(const int mo[3]) {
-  for (int a = 0; a < 3; a++)
-    if (mo[a] == 0) return a;
-  return 0;
+// Rows (stride-1 runs) whose length is a multiple of 256 bytes put the
+// rows of a large array at strides with many factors of two; a pass that
+// reads or writes such an array across rows (a tile with one element of
+// each of many rows at once) then concentrates its traffic on few memory
+// channels.  Measured on MI355X for a C2C pass over 512^3 FP32 along the
+// 4 KiB-strided axis: 4.9 TB/s dense, 5.8 TB/s with rows padded by 128
+// bytes; along the 2 MiB-strided axis 3.6 TB/s dense, 4.5 TB/s with the
+// middle axis padded.  Intermediate arrays of at least g_pad_min bytes get
+// 128 bytes of padding per row.
+static size_t g_pad_min = (size_t)32 << 20;
+void set_pad_threshold(size_t bytes) { g_pad_min = bytes; }
+
+static int padded_pitch(int prec, const Layout &l) {
+  const int a = stride1_axis(l.mo);
+  const size_t esz = elem_bytes(prec, l.dt);
+  const size_t row = (size_t)l.dims[a] * esz;
+  if (row % 256 != 0 || layout_elems(l) * esz < g_pad_min) return 0;
+  return l.dims[a] + (int)(128 / esz);
 }
+
 
 // Radix plan for length n: 8s (with one 16 or 4 absorbing the remainder) for
 // the power-of-two part, then odd primes up to maxp, largest first.  Returns
@@ -561,7 +585,7 @@ bool describe_pass(int prec, int axis, const OpSpec &op, const Layout &lin, cons
   ps.in_bytes = layout_elems(lin) * elem_bytes(prec, lin.dt);
   ps.out_bytes = layout_elems(lout) * elem_bytes(prec, lout.dt);
   ps.inplace_ok = (lin.mo[0] == lout.mo[0] && lin.mo[1] == lout.mo[1] && lin.mo[2] == lout.mo[2] &&
-                   lin.dt == lout.dt && nin == nout);
+                   lin.dt == lout.dt && nin == nout && is[0] == os[0] && is[1] == os[1] && is[2] == os[2]);
   ps.deriv_capable = (op.cls != CLS_EMPTY) && lout.dt == 2;
   // compute_deriv_loc (build/exec.C:228-287): g = 2(n-1) after R2C, else n
   a.dg = (op.cls == CLS_R2C) ? 2 * (nout - 1) : nout;
@@ -670,8 +694,13 @@ bool plan_chain(int prec, const OpSpec ops[3], const int order[3], const Layout 
       best = sel;
     }
   }
-  for (int i = 1; i < k; i++) memcpy(lay[i].mo, PERMS[best[i]], sizeof(lay[i].mo));
+  for (int i = 1; i < k; i++) {
+    memcpy(lay[i].mo, PERMS[best[i]], sizeof(lay[i].mo));
+    lay[i].pitch = 0;
+    lay[i].pitch = padded_pitch(prec, lay[i]);
+  }
   memcpy(lay[k].mo, out.mo, sizeof(lay[k].mo));
+  lay[k].pitch = out.pitch;
   char buf[256];
   for (int i = 0; i < k; i++) {
     PassDesc ps;
@@ -687,10 +716,12 @@ bool plan_chain(int prec, const OpSpec ops[3], const int order[3], const Layout 
                ps.args.bluestein ? (ps.args.mid ? ", bluestein mid" : ", bluestein")
                                  : (ps.args.fast ? ", pow2" : (ps.args.mid ? ", mid" : "")));
     if (ps.args.pair) strncat(kind, " pair", sizeof(kind) - strlen(kind) - 1);
-    snprintf(buf, sizeof(buf), "%s%s[axis %d, N=%d%s, mo (%d%d%d)->(%d%d%d), B=%d]", i ? " | " : "",
+    char pitch[24] = "";
+    if (lay[i + 1].pitch > 0) snprintf(pitch, sizeof(pitch), ", out pitch %d", lay[i + 1].pitch);
+    snprintf(buf, sizeof(buf), "%s%s[axis %d, N=%d%s, mo (%d%d%d)->(%d%d%d), B=%d%s]", i ? " | " : "",
              cls_name(ops[axes[i]].cls), axes[i], ps.args.N, kind,
              lay[i].mo[0], lay[i].mo[1], lay[i].mo[2], lay[i + 1].mo[0], lay[i + 1].mo[1], lay[i + 1].mo[2],
-             ps.args.B);
+             ps.args.B, pitch);
     chain.desc += buf;
     chain.passes.push_back(ps);
   }

@@ -24,11 +24,14 @@ struct OpSpec {
 };
 
 // An array: logical dims, MemOrder (storage rank of each logical axis; 0 is
-// stride-1), and element type (1 real, 2 complex).
+// stride-1), element type (1 real, 2 complex), and the allocated extent of
+// the stride-1 axis (pitch; 0 or at most its dim: dense).  Caller arrays are
+// dense; the planner pads the intermediate arrays between passes.
 struct Layout {
   int dims[3];
   int mo[3];
   int dt;
+  int pitch = 0;
 };
 
 struct PassDesc {
@@ -47,6 +50,10 @@ struct PassDesc {
 
 // Disable the power-of-two path (tests compare both paths).
 void set_fast_path(bool enable);
+// Intermediate arrays of at least this many bytes whose stride-1 rows are a
+// multiple of 256 bytes get padded rows (tests lower it to reach small
+// grids).
+void set_pad_threshold(size_t bytes);
 
 struct PassChain {
   std::vector<PassDesc> passes;

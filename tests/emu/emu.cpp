@@ -245,6 +245,7 @@ const char *emu_error(void) { return last_error(); }
 // 1: plan with the power-of-two path where eligible (the product default);
 // 0: generic path only.
 void emu_set_fast(int enable) { set_fast_path(enable != 0); }
+void emu_set_pad_threshold(long long bytes) { set_pad_threshold((size_t)bytes); }
 
 // Buffer assignment of a plan, for tests: dst[i] per pass, scratch sizes.
 int emu_assign(int prec, const int *cls, const int *sign, const int *kind, const int *idt, const int *odt,

@@ -28,6 +28,12 @@ def lib():
     return _lib
 
 
+def set_pad_threshold(nbytes):
+    """Intermediate arrays of at least nbytes get padded rows (the planner's
+    default is 32 MiB; 0 pads every eligible intermediate)."""
+    lib().emu_set_pad_threshold(ctypes.c_longlong(int(nbytes)))
+
+
 def ops_for(types):
     cls, sign, kind, idt, odt = [], [], [], [], []
     for t in types:

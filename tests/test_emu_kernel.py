@@ -317,3 +317,36 @@ def test_r2c_tile_load_real_pairs(dims, mo1, mo2, prec):
     got, desc = emu.exec3d(tb, od, mo2, dims, mo1, z if prec == 8 else z.astype(np.complex64), prec)
     assert "pow2" in desc.split(" | ")[-1], desc
     assert C.rel_l2(got, orc.exec3d(bwd, od, mo2, mo1, z)) < (1e-12 if prec == 8 else 1e-5), desc
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+@pytest.mark.parametrize("mo1,mo2", [([2, 1, 0], [1, 2, 0]), ([0, 1, 2], [0, 1, 2]), ([1, 0, 2], [2, 0, 1])])
+def test_padded_intermediate_arrays(prec, mo1, mo2):
+    """Intermediate arrays with padded rows (the planner pads large ones;
+    here every eligible one): rows of 32 FP32 or 16 FP64 complex values are
+    a multiple of 256 bytes and get 128 bytes of padding.  Forward R2C and
+    C2C, backward C2R, against the oracle."""
+    emu.set_pad_threshold(0)
+    try:
+        s = "D" if prec == 8 else "S"
+        n = 32 if prec == 4 else 16
+        dims = [2 * (n - 1), n, n]  # R2C x: nh = n
+        x = np.random.default_rng(prec).uniform(-1, 1, orc.storage_shape(dims, mo1))
+        tf = [C.R2C_D, C.CF_D, C.CF_D] if prec == 8 else [C.R2C_S, C.CF_S, C.CF_S]
+        tb = [C.C2R_D, C.CB_D, C.CB_D] if prec == 8 else [C.C2R_S, C.CB_S, C.CB_S]
+        sd = [n, n, n]
+        ref = orc.exec3d([C.R2C_D, C.CF_D, C.CF_D], dims, mo1, mo2, x)
+        spec, desc = emu.exec3d(tf, dims, mo1, sd, mo2, x, prec)
+        assert "pitch" in desc, desc
+        assert C.rel_l2(spec, ref) < TOL[prec], desc
+        back, desc_b = emu.exec3d(tb, sd, mo2, dims, mo1, spec, prec)
+        assert C.rel_l2(back / np.prod(dims), x) < (1e-13 if prec == 8 else 1e-6), desc_b
+        xc = (np.random.default_rng(5).standard_normal(orc.storage_shape(sd, mo1)) +
+              1j * np.random.default_rng(6).standard_normal(orc.storage_shape(sd, mo1)))
+        cf = [C.CF_D] * 3 if prec == 8 else [C.CF_S] * 3
+        got, desc_c = emu.exec3d(cf, sd, mo1, sd, mo2, xc, prec)
+        ref_c = orc.exec3d([C.CF_D] * 3, sd, mo1, mo2, xc)
+        assert C.rel_l2(got, ref_c) < TOL[prec], desc_c
+    finally:
+        emu.set_pad_threshold(32 << 20)
+
