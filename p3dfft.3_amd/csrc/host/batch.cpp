@@ -1,6 +1,10 @@
-// Batched execution: many independent transforms, one step-kernel launch per
-// stage for all of them (optionally replayed from a hipGraph), with tile
-// shapes and radix orders of the generic-path passes chosen by an autotuner.
+// Batched execution: many independent transforms of different sizes.  When
+// every item's passes run on the power-of-two or single-buffer kernels,
+// each stage is one grouped launch per pass kernel over all items' tiles
+// (the dedicated kernels, with their paired accesses and LDS twiddles);
+// otherwise one launch of the persistent step kernel per stage, with tile
+// shapes and radix orders of its generic-path passes chosen by an
+// autotuner.  Either way the launches can be replayed from a hipGraph.
 #include "batch.h"
 
 #include <stdio.h>
@@ -16,6 +20,8 @@
 extern "C" int p3e_launch_step(const void *table, const void *work, int nwork, int prec, size_t lds,
                                unsigned int grid, hipStream_t st);
 extern "C" int p3e_step_occupancy(int prec, size_t lds);
+extern "C" int p3e_launch_group(int kind, int M, const void *tab, const void *start, int n, int prec, size_t lds,
+                                int threads, unsigned int nblocks, hipStream_t st);
 
 namespace p3e {
 
@@ -239,6 +245,120 @@ static void tune_pass(int prec, PassDesc &pd, hipStream_t st, std::vector<void *
 }
 
 // ------------------------------------------------------------------ batch
+// Plans every item's chain; false (with the batch plan untouched) if they
+// differ in their number of stages or one has none.
+static bool plan_items(BatchPlan *b, const OpSpec ops[3], const int order[3], int batch_threads) {
+  std::vector<PassChain> chains(b->nitems);
+  for (int i = 0; i < b->nitems; i++) {
+    if (!plan_chain(b->prec, ops, order, b->lin[i], b->lout[i], chains[i], batch_threads)) return false;
+    if (chains[i].copy_only || chains[i].passes.empty()) {
+      set_error("batch: item %d has no transform stage", i);
+      return false;
+    }
+    if (i > 0 && chains[i].passes.size() != chains[0].passes.size()) {
+      set_error("batch: items differ in their number of stages");
+      return false;
+    }
+  }
+  b->chains.swap(chains);
+  b->nstages = (int)b->chains[0].passes.size();
+  return true;
+}
+
+// Grouped kernel of a standalone pass: 1 power-of-two, 2 single-buffer
+// without an odd prime above 5, 0 none.
+static int group_kind(const PassDesc &pd) {
+  if (pd.args.fast) return 1;
+  if (pd.args.mid && pd.args.mid_prime == 0) return 2;
+  return 0;
+}
+
+static bool build_groups(BatchPlan *b) {
+  b->groups.assign(b->nstages, {});
+  b->slot.assign((size_t)b->nstages * b->nitems, 0);
+  b->table.clear();
+  char buf[160];
+  for (int s = 0; s < b->nstages; s++) {
+    std::map<std::pair<int, int>, std::vector<int>> by_kernel;
+    for (int i = 0; i < b->nitems; i++) {
+      const PassDesc &pd = b->chains[i].passes[s];
+      const int kind = group_kind(pd);
+      by_kernel[{kind, kind == 1 ? pd.args.M : 0}].push_back(i);
+    }
+    for (auto &kv : by_kernel) {
+      BatchGroup g;
+      g.kind = kv.first.first;
+      g.M = kv.first.second;
+      g.threads = 0;
+      g.lds = 0;
+      g.tab0 = b->table.size();
+      g.n = (int)kv.second.size();
+      g.d_start = nullptr;
+      std::vector<uint32_t> start(1, 0);
+      unsigned long long tiles = 0;
+      for (int i : kv.second) {
+        const PassDesc &pd = b->chains[i].passes[s];
+        b->slot[(size_t)s * b->nitems + i] = b->table.size();
+        b->table.push_back(pd.args);
+        g.threads = std::max(g.threads, pd.threads);
+        g.lds = std::max(g.lds, pd.lds_bytes);
+        tiles += pd.nblocks;
+        if (tiles >= (1ull << 31)) {
+          set_error("batch: too many tiles in one stage");
+          return false;
+        }
+        start.push_back((uint32_t)tiles);
+      }
+      g.nblocks = (unsigned int)tiles;
+      if (hipMalloc((void **)&g.d_start, sizeof(uint32_t) * start.size()) != hipSuccess ||
+          hipMemcpy(g.d_start, start.data(), sizeof(uint32_t) * start.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        if (g.d_start) (void)hipFree(g.d_start);
+        set_error("batch: upload of a group's workgroup offsets failed");
+        return false;
+      }
+      b->owned.push_back(g.d_start);
+      b->groups[s].push_back(g);
+      snprintf(buf, sizeof(buf), "%sstage %d %s%d: %d items, %u tiles, %d threads, LDS %zu B",
+               b->desc.empty() ? "" : " | ", s, g.kind == 1 ? "pow2 M=" : "mid", g.kind == 1 ? g.M : 0, g.n,
+               g.nblocks, g.threads, g.lds);
+      b->desc += buf;
+    }
+  }
+  return true;
+}
+
+static bool build_step_lists(BatchPlan *b) {
+  b->slot.assign((size_t)b->nstages * b->nitems, 0);
+  b->table.resize((size_t)b->nstages * b->nitems);
+  for (int s = 0; s < b->nstages; s++)
+    for (int i = 0; i < b->nitems; i++) {
+      b->slot[(size_t)s * b->nitems + i] = (size_t)s * b->nitems + i;
+      b->table[(size_t)s * b->nitems + i] = b->chains[i].passes[s].args;
+    }
+  b->nwork.resize(b->nstages);
+  b->d_work.resize(b->nstages, nullptr);
+  char buf[128];
+  for (int s = 0; s < b->nstages; s++) {
+    std::vector<WorkItem> w;
+    size_t lds = 0;
+    for (int i = 0; i < b->nitems; i++) {
+      const PassDesc &pd = b->chains[i].passes[s];
+      for (unsigned int t = 0; t < pd.nblocks; t++) w.push_back(WorkItem{(uint32_t)(s * b->nitems + i), t});
+      lds = std::max(lds, pd.lds_bytes);
+    }
+    b->nwork[s] = (int)w.size();
+    b->lds = std::max(b->lds, lds);
+    if (hipMalloc((void **)&b->d_work[s], sizeof(WorkItem) * std::max<size_t>(1, w.size())) != hipSuccess ||
+        hipMemcpy(b->d_work[s], w.data(), sizeof(WorkItem) * w.size(), hipMemcpyHostToDevice) != hipSuccess) {
+      set_error("batch: work list upload failed");
+      return false;
+    }
+    snprintf(buf, sizeof(buf), "%sstage %d: %d tiles, LDS %zu B", s ? " | " : "", s, b->nwork[s], lds);
+    b->desc += buf;
+  }
+  return true;
+}
+
 BatchPlan *batch_create(int prec, int nitems, const OpSpec ops[3], const int order[3], const Layout *in,
                         const Layout *out, bool tune) {
   if (nitems < 1) {
@@ -248,41 +368,40 @@ BatchPlan *batch_create(int prec, int nitems, const OpSpec ops[3], const int ord
   BatchPlan *b = new BatchPlan();
   b->prec = prec;
   b->nitems = nitems;
-  b->h_table = nullptr;
+  b->grouped = false;
+  b->h_table[0] = b->h_table[1] = nullptr;
   b->d_table = nullptr;
   b->scratch = nullptr;
   b->scratch_bytes = 0;
   b->use_graph = false;
-  b->graph_ok = false;
-  b->graph = nullptr;
-  b->gexec = nullptr;
-  b->table_pending = false;
+  b->parity = 0;
+  for (int k = 0; k < 2; k++) {
+    b->graph_ok[k] = false;
+    b->graph[k] = nullptr;
+    b->gexec[k] = nullptr;
+    b->table_pending[k] = false;
+    (void)hipEventCreateWithFlags(&b->table_done[k], hipEventDisableTiming);
+  }
   b->lds = 0;
   b->grid = 1;
-  (void)hipEventCreateWithFlags(&b->table_done, hipEventDisableTiming);
-  b->chains.resize(nitems);
   b->lin.assign(in, in + nitems);
   b->lout.assign(out, out + nitems);
-  hipStream_t st = nullptr;
-  for (int i = 0; i < nitems; i++) {
-    if (!plan_chain(prec, ops, order, in[i], out[i], b->chains[i], P3E_STEP_THREADS)) {
-      batch_destroy(b);
-      return nullptr;
-    }
-    if (b->chains[i].copy_only || b->chains[i].passes.empty()) {
-      set_error("batch: item %d has no transform stage", i);
-      batch_destroy(b);
-      return nullptr;
-    }
-    if (i > 0 && b->chains[i].passes.size() != b->chains[0].passes.size()) {
-      set_error("batch: items differ in their number of stages");
-      batch_destroy(b);
-      return nullptr;
-    }
+  // grouped launches when every pass of every item, planned as a standalone
+  // pass, runs on a kernel that has a grouped form; else the step kernel
+  if (plan_items(b, ops, order, 0)) {
+    b->grouped = true;
+    for (const PassChain &c : b->chains)
+      for (const PassDesc &pd : c.passes)
+        if (!group_kind(pd)) b->grouped = false;
   }
-  b->nstages = (int)b->chains[0].passes.size();
+  if (!b->grouped && !plan_items(b, ops, order, P3E_STEP_THREADS)) {
+    batch_destroy(b);
+    return nullptr;
+  }
+  hipStream_t st = nullptr;
   // constant tables first (the tuner launches the pass, which reads them;
-  // re-tiling keeps the work length and so the tables), then tuning
+  // re-tiling keeps the work length and so the tables), then tuning (of the
+  // step kernel's passes)
   for (int i = 0; i < nitems; i++)
     for (PassDesc &pd : b->chains[i].passes) {
       const bool up = upload_pass_tables(prec, pd, b->owned);
@@ -291,7 +410,7 @@ BatchPlan *batch_create(int prec, int nitems, const OpSpec ops[3], const int ord
         batch_destroy(b);
         return nullptr;
       }
-      if (tune) tune_pass(prec, pd, st, b->owned);
+      if (tune && !b->grouped) tune_pass(prec, pd, st, b->owned);
       if (!tables_ready(pd)) {
         set_error("batch: tuning lost a constant table");
         batch_destroy(b);
@@ -322,41 +441,46 @@ BatchPlan *batch_create(int prec, int nitems, const OpSpec ops[3], const int ord
     batch_destroy(b);
     return nullptr;
   }
-  // stage-major table and work lists
-  const size_t ntab = (size_t)b->nstages * nitems;
-  b->table.resize(ntab);
-  for (int s = 0; s < b->nstages; s++)
-    for (int i = 0; i < nitems; i++) b->table[(size_t)s * nitems + i] = b->chains[i].passes[s].args;
-  if (hipHostMalloc((void **)&b->h_table, sizeof(PassArgs) * ntab, hipHostMallocDefault) != hipSuccess ||
+  if (!(b->grouped ? build_groups(b) : build_step_lists(b))) {
+    batch_destroy(b);
+    return nullptr;
+  }
+  const size_t ntab = b->table.size();
+  if (hipHostMalloc((void **)&b->h_table[0], sizeof(PassArgs) * ntab, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void **)&b->h_table[1], sizeof(PassArgs) * ntab, hipHostMallocDefault) != hipSuccess ||
       hipMalloc((void **)&b->d_table, sizeof(PassArgs) * ntab) != hipSuccess) {
     set_error("batch: allocation of the pass table failed");
     batch_destroy(b);
     return nullptr;
   }
-  b->nwork.resize(b->nstages);
-  b->d_work.resize(b->nstages, nullptr);
-  char buf[128];
-  for (int s = 0; s < b->nstages; s++) {
-    std::vector<WorkItem> w;
-    size_t lds = 0;
-    for (int i = 0; i < nitems; i++) {
-      const PassDesc &pd = b->chains[i].passes[s];
-      for (unsigned int t = 0; t < pd.nblocks; t++) w.push_back(WorkItem{(uint32_t)(s * nitems + i), t});
-      lds = std::max(lds, pd.lds_bytes);
-    }
-    b->nwork[s] = (int)w.size();
-    b->lds = std::max(b->lds, lds);
-    if (hipMalloc((void **)&b->d_work[s], sizeof(WorkItem) * std::max<size_t>(1, w.size())) != hipSuccess ||
-        hipMemcpy(b->d_work[s], w.data(), sizeof(WorkItem) * w.size(), hipMemcpyHostToDevice) != hipSuccess) {
-      set_error("batch: work list upload failed");
-      batch_destroy(b);
-      return nullptr;
-    }
-    snprintf(buf, sizeof(buf), "%sstage %d: %d tiles, LDS %zu B", s ? " | " : "", s, b->nwork[s], lds);
-    b->desc += buf;
-  }
   b->grid = 0;
   return b;
+}
+
+// The launches of one exec (recorded into a graph when capturing).
+static int batch_launch(BatchPlan *b, hipStream_t st) {
+  for (int s = 0; s < b->nstages; s++) {
+    if (b->grouped) {
+      for (const BatchGroup &g : b->groups[s]) {
+        const int rc = p3e_launch_group(g.kind, g.M, b->d_table + g.tab0, g.d_start, g.n, b->prec, g.lds, g.threads,
+                                        g.nblocks, st);
+        if (rc) {
+          set_error("batch group launch failed: %s", hipGetErrorString((hipError_t)rc));
+          return -1;
+        }
+      }
+    } else {
+      size_t lds = 0;
+      for (int i = 0; i < b->nitems; i++) lds = std::max(lds, b->chains[i].passes[s].lds_bytes);
+      const unsigned int grid = persistent_grid(b->prec, lds, b->nwork[s]);
+      const int rc = p3e_launch_step(b->d_table, b->d_work[s], b->nwork[s], b->prec, lds, grid, st);
+      if (rc) {
+        set_error("batch step launch failed: %s", hipGetErrorString((hipError_t)rc));
+        return -1;
+      }
+    }
+  }
+  return 0;
 }
 
 int batch_exec(BatchPlan *b, void *const *in, void *const *out, hipStream_t st, bool sync) {
@@ -364,7 +488,9 @@ int batch_exec(BatchPlan *b, void *const *in, void *const *out, hipStream_t st, 
     set_error("batch exec on an invalid plan");
     return -1;
   }
-  if (b->table_pending) BCHK(hipEventSynchronize(b->table_done));
+  const int k = b->parity;
+  // staging copy k was last read by the exec two calls ago
+  if (b->table_pending[k]) BCHK(hipEventSynchronize(b->table_done[k]));
   // buffers of every stage of every item
   for (int i = 0; i < b->nitems; i++) {
     const std::vector<PassDesc> &ps = b->chains[i].passes;
@@ -381,60 +507,66 @@ int batch_exec(BatchPlan *b, void *const *in, void *const *out, hipStream_t st, 
     for (int s = 0; s < b->nstages; s++) {
       void *d = out[i];
       if (dst[s] == BUF_S0 || dst[s] == BUF_S1) {
-        const int k = 2 * i + (dst[s] - BUF_S0);
-        d = (char *)b->scratch + b->scratch_off[k];
+        const int q = 2 * i + (dst[s] - BUF_S0);
+        d = (char *)b->scratch + b->scratch_off[q];
       }
-      PassArgs &a = b->table[(size_t)s * b->nitems + i];
+      PassArgs &a = b->table[b->slot[(size_t)s * b->nitems + i]];
       a.in = src;
       a.out = d;
+      if (b->grouped) {
+        a.vec2 = ps[s].args.vec2;
+        a.vec2s = ps[s].args.vec2s;
+        drop_unaligned_pairs(a, b->prec);
+      }
       src = d;
     }
   }
-  memcpy(b->h_table, b->table.data(), sizeof(PassArgs) * b->table.size());
-  if (b->use_graph && b->graph_ok) {
-    BCHK(hipGraphLaunch(b->gexec, st));
+  memcpy(b->h_table[k], b->table.data(), sizeof(PassArgs) * b->table.size());
+  if (b->use_graph && b->graph_ok[k]) {
+    BCHK(hipGraphLaunch(b->gexec[k], st));
   } else {
-    bool capture = b->use_graph && !b->graph_ok;
+    const bool capture = b->use_graph;
     if (capture) BCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    BCHK(hipMemcpyAsync(b->d_table, b->h_table, sizeof(PassArgs) * b->table.size(), hipMemcpyHostToDevice, st));
-    for (int s = 0; s < b->nstages; s++) {
-      size_t lds = 0;
-      for (int i = 0; i < b->nitems; i++) lds = std::max(lds, b->chains[i].passes[s].lds_bytes);
-      const unsigned int grid = persistent_grid(b->prec, lds, b->nwork[s]);
-      int rc = p3e_launch_step(b->d_table, b->d_work[s], b->nwork[s], b->prec, lds, grid, st);
-      if (rc) {
-        if (capture) {
-          hipGraph_t g;
-          (void)hipStreamEndCapture(st, &g);
-        }
-        set_error("batch step launch failed: %s", hipGetErrorString((hipError_t)rc));
+    hipError_t e = hipMemcpyAsync(b->d_table, b->h_table[k], sizeof(PassArgs) * b->table.size(),
+                                  hipMemcpyHostToDevice, st);
+    int rc = e == hipSuccess ? batch_launch(b, st) : -1;
+    if (e != hipSuccess) set_error("batch table upload failed: %s", hipGetErrorString(e));
+    if (capture) {
+      hipGraph_t g = nullptr;
+      const hipError_t ee = hipStreamEndCapture(st, &g);
+      if (rc || ee != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        if (!rc) set_error("hipStreamEndCapture failed: %s", hipGetErrorString(ee));
         return -1;
       }
-    }
-    if (capture) {
-      BCHK(hipStreamEndCapture(st, &b->graph));
-      BCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
-      b->graph_ok = true;
-      BCHK(hipGraphLaunch(b->gexec, st));
+      b->graph[k] = g;
+      BCHK(hipGraphInstantiate(&b->gexec[k], b->graph[k], nullptr, nullptr, 0));
+      b->graph_ok[k] = true;
+      BCHK(hipGraphLaunch(b->gexec[k], st));
+    } else if (rc) {
+      return -1;
     }
   }
-  BCHK(hipEventRecord(b->table_done, st));
-  b->table_pending = true;
+  BCHK(hipEventRecord(b->table_done[k], st));
+  b->table_pending[k] = true;
+  b->parity = 1 - k;
   if (sync) BCHK(hipStreamSynchronize(st));
   return 0;
 }
 
 void batch_destroy(BatchPlan *b) {
   if (!b) return;
-  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (int k = 0; k < 2; k++) {
+    if (b->gexec[k]) (void)hipGraphExecDestroy(b->gexec[k]);
+    if (b->graph[k]) (void)hipGraphDestroy(b->graph[k]);
+    if (b->h_table[k]) (void)hipHostFree(b->h_table[k]);
+    if (b->table_done[k]) (void)hipEventDestroy(b->table_done[k]);
+  }
   for (void *p : b->owned) (void)hipFree(p);
   for (WorkItem *w : b->d_work)
     if (w) (void)hipFree(w);
   if (b->d_table) (void)hipFree(b->d_table);
-  if (b->h_table) (void)hipHostFree(b->h_table);
   if (b->scratch) (void)hipFree(b->scratch);
-  if (b->table_done) (void)hipEventDestroy(b->table_done);
   delete b;
 }
 

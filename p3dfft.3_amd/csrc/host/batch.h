@@ -9,18 +9,38 @@
 
 namespace p3e {
 
+// A grouped launch of a batch stage: the same pass kernel (power-of-two of
+// complex length M, kind 1; single-buffer without an odd prime above 5,
+// kind 2) over n items whose PassArgs are table entries tab0 .. tab0+n-1.
+struct BatchGroup {
+  int kind, M;
+  int threads;             // the largest of the items'
+  size_t lds;              // the largest of the items'
+  unsigned int nblocks;    // all tiles of the n items
+  size_t tab0;
+  int n;
+  uint32_t *d_start;       // n + 1 workgroup offsets (device)
+};
+
 struct BatchPlan {
   int prec;
   int nitems;
   int nstages;                    // passes per item (the same for all items)
+  // grouped: items planned as standalone passes, each stage one grouped
+  // launch per pass kernel; else every stage one step-kernel launch
+  bool grouped;
   std::vector<Layout> lin, lout;  // per item
   std::vector<PassChain> chains;  // per item
   std::vector<void *> owned;      // device constant tables
-  std::vector<PassArgs> table;    // stage-major: table[s * nitems + i]
-  PassArgs *h_table;              // pinned staging copy of `table`
+  std::vector<PassArgs> table;    // the launches' PassArgs
+  std::vector<size_t> slot;       // table index of stage s of item i: slot[s * nitems + i]
+  std::vector<std::vector<BatchGroup>> groups;  // per stage (grouped)
+  // two pinned staging copies of `table`, used by alternate execs so that
+  // the host fills one while the device may still read the other
+  PassArgs *h_table[2];
   PassArgs *d_table;
-  std::vector<WorkItem *> d_work;  // per stage
-  std::vector<int> nwork;          // per stage
+  std::vector<WorkItem *> d_work;  // per stage (step kernel)
+  std::vector<int> nwork;          // per stage (step kernel)
   size_t lds;                      // dynamic LDS of the step kernel
   unsigned int grid;               // workgroups per step launch
   std::vector<size_t> scratch_off;  // per item: two scratch slots
@@ -28,11 +48,12 @@ struct BatchPlan {
   void *scratch;
   size_t scratch_bytes;
   bool use_graph;
-  hipGraph_t graph;
-  hipGraphExec_t gexec;
-  bool graph_ok;
-  hipEvent_t table_done;  // the last table upload has been consumed
-  bool table_pending;
+  hipGraph_t graph[2];  // one per staging copy
+  hipGraphExec_t gexec[2];
+  bool graph_ok[2];
+  hipEvent_t table_done[2];  // the upload from staging copy k has been made
+  bool table_pending[2];
+  int parity;                // staging copy of the next exec
   std::string desc;
 };
 

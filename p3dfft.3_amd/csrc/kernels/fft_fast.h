@@ -174,8 +174,7 @@ P3E_HD P3E_INL void fast_pair_index(const PassArgs &p, int i, int *j, int *k) {
 //   Z_k = (X_k + conj X_{M-k}) + i e^{+2 pi i k/N} (X_k - conj X_{M-k});
 // the imaginary parts of X_0 and X_M do not contribute (FFTW c2r).
 template <typename T, int M>
-P3E_HD P3E_INL void c2r_pair_put(const PassArgs &p, cx<T> *line, int k, cx<T> xk, cx<T> xm) {
-  const cx<T> *wN = reinterpret_cast<const cx<T> *>(p.wN);
+P3E_HD P3E_INL void c2r_pair_put(const cx<T> *wN, cx<T> *line, int k, cx<T> xk, cx<T> xm) {
   if (k == 0) {
     line[P3E_PAD(0)] = mk(xk.x + xm.x, xk.x - xm.x);
     return;
@@ -312,7 +311,7 @@ P3E_HD P3E_INL void fast_fetch(const PassArgs &p, uint32_t f, int e0, int nb, Ti
 
 template <typename T, int M, bool FULL>
 P3E_HD P3E_INL void fast_put(const PassArgs &p, uint32_t f, int e0, int nb, const TileRegs<T> &R, cx<T> *buf,
-                             int tid, int nth) {
+                             int tid, int nth, const cx<T> *wN) {
   constexpr int E = P3E_FAST_E;
   constexpr int logM = Log2<M>::v;
   const int tot = (p.load_order == ORD_LINE ? nb : p.B) * p.nin;
@@ -374,7 +373,7 @@ P3E_HD P3E_INL void fast_put(const PassArgs &p, uint32_t f, int e0, int nb, cons
       int j, k;
       fast_pair_index<M>(p, i, &j, &k);
       if (FULL || j < nb)
-        c2r_pair_put<T, M>(p, buf + j * lp, k, mk(R.v[4 * q], R.v[4 * q + 1]), mk(R.v[4 * q + 2], R.v[4 * q + 3]));
+        c2r_pair_put<T, M>(wN, buf + j * lp, k, mk(R.v[4 * q], R.v[4 * q + 1]), mk(R.v[4 * q + 2], R.v[4 * q + 3]));
     }
     // (M/2 + 1) B pairs are B more than (E/2) nth: loaded here directly
     const char *tin = reinterpret_cast<const char *>(p.in) +
@@ -385,7 +384,7 @@ P3E_HD P3E_INL void fast_put(const PassArgs &p, uint32_t f, int e0, int nb, cons
       fast_pair_index<M>(p, i, &j, &k);
       if (j >= nb) continue;
       const int o = j * ise;
-      c2r_pair_put<T, M>(p, buf + j * lp, k, ldg<cx<T>>(tin, (uint32_t)(o + k * isd) * (uint32_t)(2 * sizeof(T))),
+      c2r_pair_put<T, M>(wN, buf + j * lp, k, ldg<cx<T>>(tin, (uint32_t)(o + k * isd) * (uint32_t)(2 * sizeof(T))),
                          ldg<cx<T>>(tin, (uint32_t)(o + (M - k) * isd) * (uint32_t)(2 * sizeof(T))));
     }
   } else {
@@ -400,10 +399,11 @@ P3E_HD P3E_INL void fast_put(const PassArgs &p, uint32_t f, int e0, int nb, cons
 }
 
 template <typename T, int M, bool FULL>
-P3E_HD P3E_INL void fast_load(const PassArgs &p, uint32_t f, int e0, int nb, cx<T> *buf, int tid, int nth) {
+P3E_HD P3E_INL void fast_load(const PassArgs &p, uint32_t f, int e0, int nb, cx<T> *buf, int tid, int nth,
+                              const cx<T> *wN) {
   TileRegs<T> R;
   fast_fetch<T, M, FULL>(p, f, e0, nb, R, tid, nth);
-  fast_put<T, M, FULL>(p, f, e0, nb, R, buf, tid, nth);
+  fast_put<T, M, FULL>(p, f, e0, nb, R, buf, tid, nth, wN);
 }
 
 // Output rule of element k of line j (line j + e0 of the array): the
@@ -459,11 +459,11 @@ P3E_HD P3E_INL void store_pair_tile(const PassArgs &p, uint32_t f, int e0, const
 
 // Store phase: output rule and LDS -> tile.
 template <typename T, int M, bool FULL>
-P3E_HD P3E_INL void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, const cx<T> *buf, int tid, int nth) {
+P3E_HD P3E_INL void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, const cx<T> *buf, int tid, int nth,
+                               const cx<T> *wN) {
   constexpr int logM = Log2<M>::v;
   constexpr int E = P3E_FAST_E;
   constexpr uint32_t RS = sizeof(T), CS = 2 * sizeof(T);
-  const cx<T> *wN = reinterpret_cast<const cx<T> *>(p.wN);
   const uint32_t esz = p.cls == CLS_C2R ? RS : CS;
   char *tout = reinterpret_cast<char *>(p.out) + ((long long)e0 * p.os_e + (long long)f * p.os_f) * esz;
   const int osd = (int)p.os_d, ose = (int)p.os_e;
@@ -585,24 +585,26 @@ P3E_HD P3E_INL void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, co
 // Only threads tid < nth work (a batch workgroup may be larger than the
 // pass needs); the others take part in the barriers.  The branches on the
 // tile being whole and on the DFT sign are uniform over the workgroup.
+// tw, wN: the pass's twiddles and R2C/C2R split factors (in LDS or global
+// memory).
 template <typename T, int M, typename Run, typename Each>
-P3E_HD P3E_INL void fast_tile(const PassArgs &p, uint32_t tile, cx<T> *buf, int nth, Run &run, Each &each) {
+P3E_HD P3E_INL void fast_tile(const PassArgs &p, uint32_t tile, cx<T> *buf, int nth, Run &run, Each &each,
+                              const cx<T> *tw, const cx<T> *wN) {
   uint32_t f;
   int e0, nb;
   fast_tile_coords(p, tile, &f, &e0, &nb);
   const bool full = nb == p.B;
   each([&](int tid) P3E_LAMBDA_INL {
     if (tid >= nth) return;
-    if (full) fast_load<T, M, true>(p, f, e0, nb, buf, tid, nth);
-    else fast_load<T, M, false>(p, f, e0, nb, buf, tid, nth);
+    if (full) fast_load<T, M, true>(p, f, e0, nb, buf, tid, nth, wN);
+    else fast_load<T, M, false>(p, f, e0, nb, buf, tid, nth, wN);
   });
-  const cx<T> *tw = reinterpret_cast<const cx<T> *>(p.tw);
   if (p.cls == CLS_R2C || p.sign < 0) fft_pow2<T, M, -1>(run, buf, p.lp, tw);
   else fft_pow2<T, M, 1>(run, buf, p.lp, tw);
   each([&](int tid) P3E_LAMBDA_INL {
     if (tid >= nth) return;
-    if (full) fast_store<T, M, true>(p, f, e0, nb, buf, tid, nth);
-    else fast_store<T, M, false>(p, f, e0, nb, buf, tid, nth);
+    if (full) fast_store<T, M, true>(p, f, e0, nb, buf, tid, nth, wN);
+    else fast_store<T, M, false>(p, f, e0, nb, buf, tid, nth, wN);
   });
 }
 

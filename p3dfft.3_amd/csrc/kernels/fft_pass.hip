@@ -52,7 +52,8 @@ template <typename T, int M>
 __device__ void step_pow2(const PassArgs &p, uint32_t tile, cx<T> *buf) {
   DevPassRun<T> run{(int)threadIdx.x, p.nthreads};
   DevEach each{(int)threadIdx.x};
-  fast_tile<T, M>(p, tile, buf, p.nthreads, run, each);
+  fast_tile<T, M>(p, tile, buf, p.nthreads, run, each, reinterpret_cast<const cx<T> *>(p.tw),
+                  reinterpret_cast<const cx<T> *>(p.wN));
 }
 
 template <typename T>
@@ -119,6 +120,10 @@ int launch_pow2_pass(const PassArgs *a, int prec, size_t lds_bytes, int threads,
                      hipStream_t st);  // fft_pow2.hip
 int launch_mid_pass(const PassArgs *a, int prec, size_t lds_bytes, int threads, unsigned int nblocks,
                     hipStream_t st);  // fft_midk.hip
+int launch_pow2_group(int M, const PassArgs *tab, const uint32_t *start, int n, int prec, size_t lds_bytes,
+                      int threads, unsigned int nblocks, hipStream_t st);  // fft_pow2.hip
+int launch_mid_group(const PassArgs *tab, const uint32_t *start, int n, int prec, size_t lds_bytes, int threads,
+                     unsigned int nblocks, hipStream_t st);  // fft_midk.hip
 
 }  // namespace p3e
 
@@ -128,19 +133,9 @@ extern "C" int p3e_launch_pass(const p3e::PassArgs *a, int prec, size_t lds_byte
   if (nblocks == 0) return 0;
   if (a->fast) {
     if (threads < 1 || threads > P3E_FAST_MAX_THREADS) return (int)hipErrorInvalidValue;
-    // a pair is two reals (R2C in, C2R out) or two complex values
-    const size_t in_pair = (a->cls == p3e::CLS_R2C ? 2 : 4) * (size_t)prec;
-    const size_t out_pair = (a->cls == p3e::CLS_C2R ? 2 : 4) * (size_t)prec;
-    const bool in_odd = a->vec2 && (reinterpret_cast<uintptr_t>(a->in) % in_pair) != 0;
-    const bool out_odd = a->vec2s && (reinterpret_cast<uintptr_t>(a->out) % out_pair) != 0;
-    if (in_odd || out_odd) {
-      // the paired loads / stores need the array aligned to one pair
-      p3e::PassArgs b = *a;
-      if (in_odd) b.vec2 = 0;
-      if (out_odd) b.vec2s = 0;
-      return p3e::launch_pow2_pass(&b, prec, lds_bytes, threads, nblocks, st);
-    }
-    return p3e::launch_pow2_pass(a, prec, lds_bytes, threads, nblocks, st);
+    p3e::PassArgs b = *a;
+    p3e::drop_unaligned_pairs(b, prec);
+    return p3e::launch_pow2_pass(&b, prec, lds_bytes, threads, nblocks, st);
   }
   if (a->mid) {
     if (threads < 1 || threads > P3E_MID_THREADS(prec, a->mid_prime)) return (int)hipErrorInvalidValue;
@@ -158,6 +153,26 @@ extern "C" int p3e_launch_pass(const p3e::PassArgs *a, int prec, size_t lds_byte
     hipLaunchKernelGGL(p3e::pass_kernel<float>, dim3(nblocks), dim3(threads), lds_bytes, st, *a);
   }
   return (int)hipGetLastError();
+}
+
+// One launch of the same pass of n transforms (a batch stage group): kind
+// 1 power-of-two passes of complex length M, kind 2 single-buffer passes
+// without an odd prime radix above 5.  tab (n PassArgs) and start (n + 1
+// workgroup offsets, start[n] = nblocks) are device arrays.
+extern "C" int p3e_launch_group(int kind, int M, const void *tab, const void *start, int n, int prec, size_t lds,
+                                int threads, unsigned int nblocks, hipStream_t st) {
+  if (nblocks == 0 || n <= 0) return 0;
+  const p3e::PassArgs *t = (const p3e::PassArgs *)tab;
+  const uint32_t *s = (const uint32_t *)start;
+  if (kind == 1) {
+    if (threads < 1 || threads > P3E_FAST_MAX_THREADS) return (int)hipErrorInvalidValue;
+    return p3e::launch_pow2_group(M, t, s, n, prec, lds, threads, nblocks, st);
+  }
+  if (kind == 2) {
+    if (threads < 1 || threads > P3E_MID_THREADS(prec, 0)) return (int)hipErrorInvalidValue;
+    return p3e::launch_mid_group(t, s, n, prec, lds, threads, nblocks, st);
+  }
+  return (int)hipErrorInvalidValue;
 }
 
 extern "C" int p3e_launch_step(const void *table, const void *work, int nwork, int prec, size_t lds,

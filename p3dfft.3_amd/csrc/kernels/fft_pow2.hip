@@ -16,27 +16,39 @@ namespace p3e {
 // split factors are read by the C2R load phase itself, so R2C/C2R passes
 // wait at a barrier after copying them.
 template <typename T, int M>
-__global__ void __launch_bounds__(P3E_FAST_MAX_THREADS) pass_kernel_pow2(PassArgs p0) {
+__device__ __forceinline__ void pow2_body(const PassArgs &p, uint32_t tile) {
   extern __shared__ unsigned char smem_raw[];
   cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
-  cx<T> *ltw = buf + (size_t)p0.B * p0.lp;
+  cx<T> *ltw = buf + (size_t)p.B * p.lp;
   cx<T> *lwn = ltw + M;
-  const cx<T> *gtw = reinterpret_cast<const cx<T> *>(p0.tw);
-  const cx<T> *gwn = reinterpret_cast<const cx<T> *>(p0.wN);
-  const int tid = threadIdx.x, nth = blockDim.x;
-  for (int i = tid; i < p0.ntw; i += nth) ltw[i] = gtw[i];
-  if (p0.cls != CLS_C2C) {
-    for (int i = tid; i <= M; i += nth) lwn[i] = gwn[i];
+  const cx<T> *gtw = reinterpret_cast<const cx<T> *>(p.tw);
+  const cx<T> *gwn = reinterpret_cast<const cx<T> *>(p.wN);
+  const int tid = threadIdx.x, nth = p.nthreads;
+  for (int i = tid; i < p.ntw; i += blockDim.x) ltw[i] = gtw[i];
+  if (p.cls != CLS_C2C) {
+    for (int i = tid; i <= M; i += blockDim.x) lwn[i] = gwn[i];
     // the C2R load phase merges (X_k, X_{M-k}) with the split factors, so
     // they must be in LDS before any thread's load phase
     __syncthreads();
   }
-  PassArgs p = p0;
-  p.tw = ltw;
-  p.wN = lwn;
   DevPassRun<T> run{tid, nth};
   DevEach each{tid};
-  fast_tile<T, M>(p, blockIdx.x, buf, nth, run, each);
+  fast_tile<T, M>(p, tile, buf, nth, run, each, ltw, lwn);
+}
+
+template <typename T, int M>
+__global__ void __launch_bounds__(P3E_FAST_MAX_THREADS) pass_kernel_pow2(PassArgs p) {
+  pow2_body<T, M>(p, blockIdx.x);
+}
+
+// The same pass of many transforms in one launch (a batch stage): item i
+// owns workgroups start[i] .. start[i+1] - 1 and runs with tab[i]; the
+// workgroup size is the largest nthreads of the items (the others idle).
+template <typename T, int M>
+__global__ void __launch_bounds__(P3E_FAST_MAX_THREADS)
+    pass_kernel_pow2_grp(const PassArgs *__restrict__ tab, const uint32_t *__restrict__ start, int n) {
+  const uint32_t i = group_item(start, n, blockIdx.x);
+  pow2_body<T, M>(tab[i], blockIdx.x - start[i]);
 }
 
 template <typename T, int M>
@@ -62,6 +74,41 @@ static int launch_pow2_any(const PassArgs *a, size_t lds_bytes, int threads, uns
     case 4096: return launch_pow2<T, 4096>(a, lds_bytes, threads, nblocks, st);
   }
   return (int)hipErrorInvalidValue;
+}
+
+template <typename T, int M>
+static int launch_pow2_grp(const PassArgs *tab, const uint32_t *start, int n, size_t lds_bytes, int threads,
+                           unsigned int nblocks, hipStream_t st) {
+  if (lds_bytes > 65536)
+    (void)hipFuncSetAttribute((const void *)pass_kernel_pow2_grp<T, M>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds_bytes);
+  hipLaunchKernelGGL((pass_kernel_pow2_grp<T, M>), dim3(nblocks), dim3(threads), lds_bytes, st, tab, start, n);
+  return (int)hipGetLastError();
+}
+
+template <typename T>
+static int launch_pow2_grp_any(int M, const PassArgs *tab, const uint32_t *start, int n, size_t lds_bytes,
+                               int threads, unsigned int nblocks, hipStream_t st) {
+  switch (M) {
+    case 16: return launch_pow2_grp<T, 16>(tab, start, n, lds_bytes, threads, nblocks, st);
+    case 32: return launch_pow2_grp<T, 32>(tab, start, n, lds_bytes, threads, nblocks, st);
+    case 64: return launch_pow2_grp<T, 64>(tab, start, n, lds_bytes, threads, nblocks, st);
+    case 128: return launch_pow2_grp<T, 128>(tab, start, n, lds_bytes, threads, nblocks, st);
+    case 256: return launch_pow2_grp<T, 256>(tab, start, n, lds_bytes, threads, nblocks, st);
+    case 512: return launch_pow2_grp<T, 512>(tab, start, n, lds_bytes, threads, nblocks, st);
+    case 1024: return launch_pow2_grp<T, 1024>(tab, start, n, lds_bytes, threads, nblocks, st);
+    case 2048: return launch_pow2_grp<T, 2048>(tab, start, n, lds_bytes, threads, nblocks, st);
+    case 4096: return launch_pow2_grp<T, 4096>(tab, start, n, lds_bytes, threads, nblocks, st);
+  }
+  return (int)hipErrorInvalidValue;
+}
+
+// A grouped launch of power-of-two passes of complex length M (tab and
+// start in device memory; nblocks = start[n]).
+int launch_pow2_group(int M, const PassArgs *tab, const uint32_t *start, int n, int prec, size_t lds_bytes,
+                      int threads, unsigned int nblocks, hipStream_t st) {
+  return prec == 8 ? launch_pow2_grp_any<double>(M, tab, start, n, lds_bytes, threads, nblocks, st)
+                   : launch_pow2_grp_any<float>(M, tab, start, n, lds_bytes, threads, nblocks, st);
 }
 
 // Launch of a power-of-two pass (the array alignment checks of the paired

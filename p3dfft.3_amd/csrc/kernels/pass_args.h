@@ -100,6 +100,16 @@ struct WorkItem {
   uint32_t pass, tile;
 };
 
+// The paired global accesses of the power-of-two path (vec2, vec2s) need
+// the array aligned to one pair: two reals (R2C input, C2R output) or two
+// complex values.  Clears the flag of a side whose array is not.
+static inline void drop_unaligned_pairs(PassArgs &a, int prec) {
+  const unsigned long in_pair = (a.cls == CLS_R2C ? 2ul : 4ul) * (unsigned long)prec;
+  const unsigned long out_pair = (a.cls == CLS_C2R ? 2ul : 4ul) * (unsigned long)prec;
+  if (a.vec2 && reinterpret_cast<unsigned long>(a.in) % in_pair != 0) a.vec2 = 0;
+  if (a.vec2s && reinterpret_cast<unsigned long>(a.out) % out_pair != 0) a.vec2s = 0;
+}
+
 // Workgroup size of the batch (step) kernel.
 #define P3E_STEP_THREADS 256
 

@@ -44,4 +44,17 @@ struct DevEach {
   }
 };
 
+// The item of a grouped launch that owns workgroup b: item i owns
+// workgroups start[i] .. start[i+1] - 1 (start[0] = 0, n items); a uniform
+// binary search.
+__device__ __forceinline__ uint32_t group_item(const uint32_t *start, int n, uint32_t b) {
+  uint32_t lo = 0, hi = (uint32_t)n;
+  while (hi - lo > 1) {
+    const uint32_t m = (lo + hi) >> 1;
+    if (start[m] <= b) lo = m;
+    else hi = m;
+  }
+  return lo;
+}
+
 }  // namespace p3e

@@ -55,7 +55,8 @@ template <typename T, int M> static int run_pow2(const PassDesc &pd, const PassA
   std::vector<cx<T>> buf((size_t)a.B * a.lp);
   for (unsigned int t = 0; t < pd.nblocks; t++) {
     for (auto &x : buf) x = cx<T>{(T)1e30, (T)-1e30};
-    fast_tile<T, M>(a, t, buf.data(), pd.threads, run, each);
+    fast_tile<T, M>(a, t, buf.data(), pd.threads, run, each, reinterpret_cast<const cx<T> *>(a.tw),
+                    reinterpret_cast<const cx<T> *>(a.wN));
   }
   return 0;
 }
