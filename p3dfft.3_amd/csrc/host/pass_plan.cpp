@@ -261,6 +261,12 @@ static void setup_mid(int prec, const OpSpec &op, PassDesc &ps, int batch_thread
   int th = (int)(((long)lines(B) * L + E - 1) / E);
   th = std::max(64, (th + 63) & ~63);
   if (th > maxth) return;
+  // the kernel addresses a tile's elements with 32-bit byte offsets from
+  // the tile's first element (below 2^31 on both sides)
+  const double in_esz = (double)prec * (a.in_cplx ? 2 : 1), out_esz = (double)prec * (a.out_cplx ? 2 : 1);
+  if (((double)a.nin * a.is_d + (double)B * a.is_e + 1) * in_esz >= 2147483648.0 ||
+      ((double)a.nout * a.os_d + (double)B * a.os_e + 1) * out_esz >= 2147483648.0)
+    return;
   a.mid = 1;
   a.mid_prime = prime;
   a.pair = pair;

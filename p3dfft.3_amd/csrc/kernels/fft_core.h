@@ -476,7 +476,9 @@ P3E_HD P3E_INL void generic_load(const PassArgs &p, uint32_t f, int e0, int nb, 
 
 // Output rule (chirp, post-twiddle, derivative, spectral operator, scale)
 // and store of the transformed lines in `res`.
-template <typename T>
+// OFF32: 32-bit offsets within the tile (the single-buffer path, whose
+// passes the planner admits only when they stay below 2^31 bytes).
+template <typename T, bool OFF32 = false>
 P3E_HD P3E_INL void generic_store(const PassArgs &p, uint32_t f, int e0, int nb, const cx<T> *res, int tid,
                                   int nth) {
   const cx<T> *chirp = reinterpret_cast<const cx<T> *>(p.chirp);
@@ -506,11 +508,19 @@ P3E_HD P3E_INL void generic_store(const PassArgs &p, uint32_t f, int e0, int nb,
     }
     if (p.spec_op) y = scl(y, poisson_factor<T>(p, k, e0 + j, (int)f));
     if (p.has_scale) y = scl(y, (T)p.scale);
-    const long long off = base + (long long)k * p.os_d + (long long)j * p.os_e;
-    if (p.out_cplx) {
-      reinterpret_cast<cx<T> *>(p.out)[off] = y;
+    if (OFF32) {
+      const uint32_t esz = p.out_cplx ? 2 * sizeof(T) : sizeof(T);
+      char *t = reinterpret_cast<char *>(p.out) + base * esz;
+      const uint32_t off = (uint32_t)(k * (int)p.os_d + j * (int)p.os_e) * esz;
+      if (p.out_cplx) *reinterpret_cast<cx<T> *>(t + off) = y;
+      else *reinterpret_cast<T *>(t + off) = y.x;
     } else {
-      reinterpret_cast<T *>(p.out)[off] = y.x;
+      const long long off = base + (long long)k * p.os_d + (long long)j * p.os_e;
+      if (p.out_cplx) {
+        reinterpret_cast<cx<T> *>(p.out)[off] = y;
+      } else {
+        reinterpret_cast<T *>(p.out)[off] = y.x;
+      }
     }
   }
 }

@@ -20,16 +20,18 @@
 namespace p3e {
 
 // Radix-R pass at input stride Ns over nb padded lines of length L.
-template <typename T, int R> struct MPass {
+template <typename T, int R, int S> struct MPass {
   static constexpr int NB = (P3E_MID_E + R - 1) / R;  // butterflies per thread, at most
   static constexpr int SLOTS = NB * R;
   cx<T> *buf;
-  int L, lpad, nb, Ns, s;
+  int L, lpad, nb, Ns;
   const cx<T> *tw;
   FastDiv dnbf, dns;
 
   P3E_HD P3E_INL void read(int tid, int nth, cx<T> *v) const {
     const int nbf = L / R, total = nb * nbf;
+    // P3E_PAD(j + r nbf) = P3E_PAD(j) + r (nbf + nbf / 16) when 16 divides nbf
+    const int step = nbf % 16 == 0 ? nbf + (nbf >> 4) : 0;
 #pragma unroll
     for (int m = 0; m < NB; m++) {
       const int b = tid + m * nth;
@@ -37,8 +39,14 @@ template <typename T, int R> struct MPass {
         const int line = (int)fdiv((uint32_t)b, dnbf);
         const int j = b - line * nbf;
         const cx<T> *sl = buf + line * lpad;
+        if (step) {
+          const cx<T> *q = sl + P3E_PAD(j);
 #pragma unroll
-        for (int r = 0; r < R; r++) v[m * R + r] = sl[P3E_PAD(j + r * nbf)];
+          for (int r = 0; r < R; r++) v[m * R + r] = q[r * step];
+        } else {
+#pragma unroll
+          for (int r = 0; r < R; r++) v[m * R + r] = sl[P3E_PAD(j + r * nbf)];
+        }
       }
     }
   }
@@ -56,16 +64,24 @@ template <typename T, int R> struct MPass {
         cx<T> *w = v + m * R;
         if (Ns > 1) {
 #pragma unroll
-          for (int r = 1; r < R; r++) w[r] = mul(w[r], twid(tw, k * r * tstride, s));
+          for (int r = 1; r < R; r++) w[r] = mul(w[r], twid(tw, k * r * tstride, S));
         }
         cx<T> *dl = buf + line * lpad;
         const int base = (j - k) * R + k;
         if constexpr (is_base_radix<R>) {
-          bfly<T, R>(w, s);
+          bfly<T, R>(w, S);
+          if (Ns % 16 == 0) {
+            // P3E_PAD(base + r Ns) = P3E_PAD(base) + r (Ns + Ns / 16)
+            cx<T> *q = dl + P3E_PAD(base);
+            const int step = Ns + (Ns >> 4);
 #pragma unroll
-          for (int r = 0; r < R; r++) dl[P3E_PAD(base + r * Ns)] = w[r];
+            for (int r = 0; r < R; r++) q[r * step] = w[r];
+          } else {
+#pragma unroll
+            for (int r = 0; r < R; r++) dl[P3E_PAD(base + r * Ns)] = w[r];
+          }
         } else {
-          bf_prime<T, R>(w, s, [&](int q, cx<T> x) { dl[P3E_PAD(base + q * Ns)] = x; });
+          bf_prime<T, R>(w, S, [&](int q, cx<T> x) { dl[P3E_PAD(base + q * Ns)] = x; });
         }
       }
     }
@@ -77,23 +93,23 @@ template <typename T, int R> struct MPass {
 // radix-R pass at stride Ns at entry k r L / (Ns R).  (The pass-by-pass
 // layout of the power-of-two path measured slower here on MI355X for
 // lengths with radices 3 and 5, and faster only for power-of-two L.)
-template <typename T, int P, typename Run>
-P3E_HD P3E_INL void mid_fft(Run &run, cx<T> *buf, const PassArgs &p, int nb, int s, const cx<T> *tw) {
+template <typename T, int P, int S, typename Run>
+P3E_HD P3E_INL void mid_fft(Run &run, cx<T> *buf, const PassArgs &p, int nb, const cx<T> *tw) {
   int Ns = 1;
   for (int i = 0; i < p.nrad; i++) {
     const int R = p.radix[i];
     const FastDiv dn = p.div_nbf[i], ds = p.div_ns[i];
     switch (R) {
-      case 2: { MPass<T, 2> ps{buf, p.L, p.lpad, nb, Ns, s, tw, dn, ds}; run(ps); } break;
-      case 3: { MPass<T, 3> ps{buf, p.L, p.lpad, nb, Ns, s, tw, dn, ds}; run(ps); } break;
-      case 4: { MPass<T, 4> ps{buf, p.L, p.lpad, nb, Ns, s, tw, dn, ds}; run(ps); } break;
-      case 5: { MPass<T, 5> ps{buf, p.L, p.lpad, nb, Ns, s, tw, dn, ds}; run(ps); } break;
-      case 8: { MPass<T, 8> ps{buf, p.L, p.lpad, nb, Ns, s, tw, dn, ds}; run(ps); } break;
-      case 16: { MPass<T, 16> ps{buf, p.L, p.lpad, nb, Ns, s, tw, dn, ds}; run(ps); } break;
+      case 2: { MPass<T, 2, S> ps{buf, p.L, p.lpad, nb, Ns, tw, dn, ds}; run(ps); } break;
+      case 3: { MPass<T, 3, S> ps{buf, p.L, p.lpad, nb, Ns, tw, dn, ds}; run(ps); } break;
+      case 4: { MPass<T, 4, S> ps{buf, p.L, p.lpad, nb, Ns, tw, dn, ds}; run(ps); } break;
+      case 5: { MPass<T, 5, S> ps{buf, p.L, p.lpad, nb, Ns, tw, dn, ds}; run(ps); } break;
+      case 8: { MPass<T, 8, S> ps{buf, p.L, p.lpad, nb, Ns, tw, dn, ds}; run(ps); } break;
+      case 16: { MPass<T, 16, S> ps{buf, p.L, p.lpad, nb, Ns, tw, dn, ds}; run(ps); } break;
       default:
         // the planner admits no other radix to this instantiation
         if constexpr (P > 0) {
-          MPass<T, P> ps{buf, p.L, p.lpad, nb, Ns, s, tw, dn, ds};
+          MPass<T, P, S> ps{buf, p.L, p.lpad, nb, Ns, tw, dn, ds};
           run(ps);
         }
         break;
@@ -130,7 +146,11 @@ P3E_HD P3E_INL void mid_load(const PassArgs &p, uint32_t f, int e0, int nb, cx<T
   constexpr int U = 4;  // loads in flight per thread
   const cx<T> *chirp = reinterpret_cast<const cx<T> *>(p.chirp);
   const cx<T> *pre = reinterpret_cast<const cx<T> *>(p.pre);
-  const long long base = (long long)e0 * p.is_e + (long long)f * p.is_f;
+  // the tile's elements at 32-bit offsets from its first one (the planner
+  // admits a pass to this path only when they stay below 2^31 bytes)
+  const uint32_t esz = p.in_cplx ? 2 * sizeof(T) : sizeof(T);
+  const char *tin = reinterpret_cast<const char *>(p.in) + ((long long)e0 * p.is_e + (long long)f * p.is_f) * esz;
+  const int isd = (int)p.is_d, ise = (int)p.is_e;
   const bool lo = p.load_order == ORD_LINE;
   const int tot = (lo ? nb : p.B) * p.nin;
   for (int i0 = tid; i0 < tot; i0 += U * nth) {
@@ -140,13 +160,13 @@ P3E_HD P3E_INL void mid_load(const PassArgs &p, uint32_t f, int e0, int nb, cx<T
       const int i = i0 + u * nth;
       int j, n;
       mid_load_index(lo, i, p.nin, p.div_nin, p.logB, &j, &n);
-      const long long off = (i < tot && j < nb) ? base + (long long)n * p.is_d + (long long)j * p.is_e : base;
+      const uint32_t off = (i < tot && j < nb) ? (uint32_t)(n * isd + j * ise) * esz : 0u;
       if (p.in_cplx) {
-        const cx<T> x = reinterpret_cast<const cx<T> *>(p.in)[off];
+        const cx<T> x = *reinterpret_cast<const cx<T> *>(tin + off);
         v[2 * u] = x.x;
         v[2 * u + 1] = x.y;
       } else {
-        v[2 * u] = reinterpret_cast<const T *>(p.in)[off];
+        v[2 * u] = *reinterpret_cast<const T *>(tin + off);
         v[2 * u + 1] = (T)0;
       }
     }
@@ -176,7 +196,9 @@ template <typename T>
 P3E_HD P3E_INL void pair_load(const PassArgs &p, uint32_t f, int e0, int nb, cx<T> *buf, int tid, int nth) {
   constexpr int U = 4;  // entries (line pairs at one n) loaded together, as in mid_load
   const cx<T> *chirp = reinterpret_cast<const cx<T> *>(p.chirp);
-  const long long base = (long long)e0 * p.is_e + (long long)f * p.is_f;
+  const uint32_t esz = p.cls == CLS_R2C ? sizeof(T) : 2 * sizeof(T);
+  const char *tin = reinterpret_cast<const char *>(p.in) + ((long long)e0 * p.is_e + (long long)f * p.is_f) * esz;
+  const int isd = (int)p.is_d, ise = (int)p.is_e;
   const int nl = (nb + 1) / 2, N = p.N;
   const int lbl = p.logB > 0 ? p.logB - 1 : 0;  // B / 2 line pairs per row (1 if B = 1)
   const bool lo = p.load_order == ORD_LINE;
@@ -190,16 +212,14 @@ P3E_HD P3E_INL void pair_load(const PassArgs &p, uint32_t f, int e0, int nb, cx<
       int t, n;
       mid_load_index(lo, i, p.nin, p.div_nin, lbl, &t, &n);
       const bool ok = i < tot && t < nl, two = ok && 2 * t + 1 < nb;
-      const long long off = ok ? base + (long long)n * p.is_d + (long long)(2 * t) * p.is_e : base;
-      const long long off2 = two ? off + p.is_e : base;
+      const uint32_t off = ok ? (uint32_t)(n * isd + 2 * t * ise) * esz : 0u;
+      const uint32_t off2 = two ? off + (uint32_t)ise * esz : 0u;
       if (p.cls == CLS_R2C) {
-        const T *in = reinterpret_cast<const T *>(p.in);
-        const T a = in[off], b = in[off2];
+        const T a = *reinterpret_cast<const T *>(tin + off), b = *reinterpret_cast<const T *>(tin + off2);
         v[4 * u] = a;
         v[4 * u + 1] = two ? b : (T)0;
       } else {
-        const cx<T> *in = reinterpret_cast<const cx<T> *>(p.in);
-        const cx<T> a = in[off], b = in[off2];
+        const cx<T> a = *reinterpret_cast<const cx<T> *>(tin + off), b = *reinterpret_cast<const cx<T> *>(tin + off2);
         v[4 * u] = a.x;
         v[4 * u + 1] = a.y;
         v[4 * u + 2] = two ? b.x : (T)0;
@@ -232,7 +252,9 @@ template <typename T>
 P3E_HD P3E_INL void pair_store(const PassArgs &p, uint32_t f, int e0, int nb, const cx<T> *res, int tid,
                                int nth) {
   const cx<T> *chirp = reinterpret_cast<const cx<T> *>(p.chirp);
-  const long long base = (long long)e0 * p.os_e + (long long)f * p.os_f;
+  const uint32_t esz = p.out_cplx ? 2 * sizeof(T) : sizeof(T);
+  char *tout = reinterpret_cast<char *>(p.out) + ((long long)e0 * p.os_e + (long long)f * p.os_f) * esz;
+  const int osd = (int)p.os_d, ose = (int)p.os_e;
   const int N = p.N;
   const int tot = (p.store_order == ORD_LINE ? nb : p.B) * p.nout;
   for (int i = tid; i < tot; i += nth) {
@@ -246,7 +268,7 @@ P3E_HD P3E_INL void pair_store(const PassArgs &p, uint32_t f, int e0, int nb, co
       if (j >= nb) continue;
     }
     const cx<T> *line = res + (j >> 1) * p.lpad;
-    const long long off = base + (long long)k * p.os_d + (long long)j * p.os_e;
+    const uint32_t off = (uint32_t)(k * osd + j * ose) * esz;
     cx<T> z = line[P3E_PAD(k)];
     if (p.bluestein) z = mul(z, chirp[k]);
     cx<T> y;
@@ -268,8 +290,8 @@ P3E_HD P3E_INL void pair_store(const PassArgs &p, uint32_t f, int e0, int nb, co
     }
     if (p.spec_op) y = scl(y, poisson_factor<T>(p, k, e0 + j, (int)f));
     if (p.has_scale) y = scl(y, (T)p.scale);
-    if (p.out_cplx) reinterpret_cast<cx<T> *>(p.out)[off] = y;
-    else reinterpret_cast<T *>(p.out)[off] = y.x;
+    if (p.out_cplx) *reinterpret_cast<cx<T> *>(tout + off) = y;
+    else *reinterpret_cast<T *>(tout + off) = y.x;
   }
 }
 
@@ -295,12 +317,13 @@ P3E_HD P3E_INL void mid_tile(const PassArgs &p, uint32_t tile, cx<T> *buf, int n
   });
   if (p.cls != CLS_EMPTY) {
     if (!p.bluestein) {
-      mid_fft<T, P>(run, buf, p, nl, p.sign, tw);
+      if (p.sign < 0) mid_fft<T, P, -1>(run, buf, p, nl, tw);
+      else mid_fft<T, P, 1>(run, buf, p, nl, tw);
     } else {
       // Bluestein: forward FFT of the chirped sequence, multiply by the
       // spectrum of the conjugate chirp, inverse FFT (the store applies the
       // output chirp)
-      mid_fft<T, P>(run, buf, p, nl, -1, tw);
+      mid_fft<T, P, -1>(run, buf, p, nl, tw);
       const cx<T> *chirp_hat = reinterpret_cast<const cx<T> *>(p.chirp_hat);
       each([&](int tid) {
         if (tid < nth)
@@ -310,13 +333,13 @@ P3E_HD P3E_INL void mid_tile(const PassArgs &p, uint32_t tile, cx<T> *buf, int n
             buf[line * p.lpad + P3E_PAD(m)] = mul(buf[line * p.lpad + P3E_PAD(m)], chirp_hat[m]);
           }
       });
-      mid_fft<T, P>(run, buf, p, nl, +1, tw);
+      mid_fft<T, P, 1>(run, buf, p, nl, tw);
     }
   }
   each([&](int tid) {
     if (tid < nth) {
       if (p.pair) pair_store<T>(p, f, e0, nb, buf, tid, nth);
-      else generic_store<T>(p, f, e0, nb, buf, tid, nth);
+      else generic_store<T, true>(p, f, e0, nb, buf, tid, nth);
     }
   });
 }
