@@ -298,6 +298,8 @@ static void setup_fast(int prec, const OpSpec &op, PassDesc &ps, int batch_threa
   int rad[P3E_MAX_RADIX_PASSES];
   const int nrad = fast_radices(M, rad);
   if (nrad == 0) return;
+  // the batch step kernel has only the power-of-two lengths
+  if (batch_threads > 0 && !is_pow2(M)) return;
   const size_t esz = 2 * (size_t)prec;
   // Line pitch in LDS: room for the padded line (P3E_PAD(M-1) < M + M/16),
   // rounded up to 1 mod 32 complex elements.  The load and store phases of
@@ -318,9 +320,19 @@ static void setup_fast(int prec, const OpSpec &op, PassDesc &ps, int batch_threa
   int Bcap = batch_threads > 0 ? 256 : 64;
   if (batch_threads == 0 && line_both) Bcap = std::max(1, 256 * P3E_FAST_E / M);
   const int Bmax = std::min(Bcap, next_pow2(a.ne));
-  // the pass kernel keeps its twiddles (M) and split factors (M + 1) in LDS
-  // after the tile; the batch kernel reads them from global memory
-  const size_t tables = batch_threads > 0 ? 0 : (size_t)(M + (op.cls == CLS_C2C ? 0 : M + 1)) * esz;
+  // the pass kernel keeps its twiddles (the pass-by-pass table: M entries
+  // for power-of-two radices, more with a radix 3 or 5, whose tw_stride
+  // exceeds R - 1) and split factors (M + 1) in LDS after the tile; the
+  // batch kernel reads them from global memory
+  size_t ntw = 1;
+  {
+    long long ns = 1;
+    for (int i = 0; i < nrad; i++) {
+      ntw += (size_t)ns * tw_stride(rad[i]);
+      ns *= rad[i];
+    }
+  }
+  const size_t tables = batch_threads > 0 ? 0 : (ntw + (op.cls == CLS_C2C ? 0 : M + 1)) * esz;
   auto lds = [&](int b) { return (size_t)b * lp * esz + tables; };
   // (a 160 KiB budget, B = 64 lines of 256 or 32 of 512 with one workgroup
   // per CU, measured slower on MI355X: 0.31 ms vs 0.27 ms for the R2C pass
@@ -384,7 +396,7 @@ static void setup_fast(int prec, const OpSpec &op, PassDesc &ps, int batch_threa
   a.nthreads = ps.threads;
   ps.nblocks = (unsigned int)((unsigned long long)a.ntile_e * (unsigned long long)a.nf);
   pass_twiddles(rad, nrad, ps.tw);
-  a.ntw = (int)ps.tw.size();  // M (power-of-two radices): the kernel puts the split factors after it
+  a.ntw = (int)ps.tw.size();  // ntw entries; the kernel puts the split factors after them
   ps.wN.clear();
   if (op.cls != CLS_C2C) {
     ps.wN.resize(M + 1);

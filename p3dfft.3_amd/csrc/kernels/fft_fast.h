@@ -1,10 +1,11 @@
-// Power-of-two path of the fused reorder + 1D transform pass.
+// Compile-time-length path of the fused reorder + 1D transform pass
+// (powers of two, and 3 or 5 times a power of two).
 //
 // Same tile as the generic path (B lines along the transform axis, adjacent
 // along the tile axis), but:
 //   - the complex length M and its radix sequence are compile-time, one
-//     kernel per M, so every index is a shift and the values a thread owns
-//     stay in registers;
+//     kernel per M, so every index is a shift (or a multiply-high by a
+//     constant) and the values a thread owns stay in registers;
 //   - one LDS buffer: each radix pass reads the P3E_FAST_E elements a thread
 //     owns into registers (read half), barrier, twiddles and butterflies and
 //     writes them to their Stockham positions (write half), barrier;
@@ -24,21 +25,29 @@ template <> struct Log2<1> {
   static constexpr int v = 0;
 };
 
+// Quotient and remainder by a compile-time divisor D (unsigned, so a power
+// of two is a shift and a mask, other divisors a multiply-high).
+template <int D> P3E_HD P3E_INL int qdiv(int i) { return (int)((uint32_t)i / (uint32_t)D); }
+template <int D> P3E_HD P3E_INL int qmod(int i) { return (int)((uint32_t)i % (uint32_t)D); }
+
 // Radix-R Stockham pass at stride Ns of a length-M FFT with DFT sign S,
 // split in halves.  Every thread owns NB butterflies of the B lines of the
 // tile (the planner makes nth * E = B * M), including lines past the end
 // of a partial tile: their LDS slots hold stale values, transformed and
-// never stored, so no butterfly needs a bounds check.
+// never stored, so no butterfly needs a bounds check when R divides E.  An
+// odd radix R (M = 3 or 5 times a power of two) gives each thread
+// ceil(E / R) butterfly slots, the last ones past the tile's B M / R
+// butterflies (skipped).
 template <typename T, int R, int M, int Ns, int S> struct FPass {
   static constexpr int E = P3E_FAST_E;
-  static constexpr int SLOTS = E;  // registers per thread between the halves
-  static constexpr int NB = E / R;  // butterflies per thread
-  static constexpr int logR = Log2<R>::v;
-  static constexpr int lognbf = Log2<M>::v - logR;
+  static constexpr int NB = (E + R - 1) / R;  // butterflies per thread
+  static constexpr bool EXACT = E % R == 0;
+  static constexpr int SLOTS = NB * R;  // registers per thread between the halves
   static constexpr int nbf = M / R;
   cx<T> *buf;
   int lp;
   const cx<T> *tw;
+  int nbt;  // butterflies of the tile: B nbf
 
   // Padded LDS positions as a per-thread base plus a compile-time offset per
   // r (so the compiler folds the offsets into the LDS instructions):
@@ -53,11 +62,18 @@ template <typename T, int R, int M, int Ns, int S> struct FPass {
 #pragma unroll
     for (int m = 0; m < NB; m++) {
       const int b = tid + m * nth;
-      const int j = b & (nbf - 1);
+      if (!EXACT && b >= nbt) continue;
+      const int j = qmod<nbf>(b);
       // elements j + r nbf, j < nbf
-      const cx<T> *sl = buf + (b >> lognbf) * lp + (nbf % 16 == 0 ? P3E_PAD(j) : j);
+      if constexpr (nbf % 16 == 0 || 16 % nbf == 0) {
+        const cx<T> *sl = buf + qdiv<nbf>(b) * lp + (nbf % 16 == 0 ? P3E_PAD(j) : j);
 #pragma unroll
-      for (int r = 0; r < R; r++) v[m * R + r] = sl[pad_off<nbf>(r)];
+        for (int r = 0; r < R; r++) v[m * R + r] = sl[pad_off<nbf>(r)];
+      } else {
+        const cx<T> *sl = buf + qdiv<nbf>(b) * lp;
+#pragma unroll
+        for (int r = 0; r < R; r++) v[m * R + r] = sl[P3E_PAD(j + r * nbf)];
+      }
     }
   }
 
@@ -65,17 +81,18 @@ template <typename T, int R, int M, int Ns, int S> struct FPass {
 #pragma unroll
     for (int m = 0; m < NB; m++) {
       const int b = tid + m * nth;
-      const int j = b & (nbf - 1);
-      const int k = j & (Ns - 1);
+      if (!EXACT && b >= nbt) continue;
+      const int j = qmod<nbf>(b);
+      const int k = qmod<Ns>(j);
       cx<T> *w = v + m * R;
       if (Ns > 1) {
 #pragma unroll
         for (int r = 1; r < R; r++) w[r] = mul(w[r], twid(tw, k * tw_stride(R) + r - 1, S));
       }
       bfly<T, R>(w, S);
-      cx<T> *dl = buf + (b >> lognbf) * lp;
+      cx<T> *dl = buf + qdiv<nbf>(b) * lp;
       // outputs hi + k + r Ns, hi = (j - k) R a multiple of Ns R, k < Ns
-      const int hi = (j - k) << logR;
+      const int hi = (j - k) * R;
       if constexpr ((Ns * R) % 16 == 0) {
         cx<T> *d = dl + (Ns % 16 == 0 ? P3E_PAD(hi + k) : P3E_PAD(hi) + k);
 #pragma unroll
@@ -93,24 +110,34 @@ template <typename T, int R, int M, int Ns, int S> struct FPass {
 // this pass onward, pass by pass (pass_twiddles on the host): Ns
 // tw_stride(R) entries for a radix-R pass at stride Ns.
 template <typename T, int M, int Ns, int S, typename Run, int R, int... Rest>
-P3E_HD P3E_INL void fft_seq(Run &run, cx<T> *buf, int lp, const cx<T> *tw) {
-  FPass<T, R, M, Ns, S> ps{buf, lp, tw};
+P3E_HD P3E_INL void fft_seq(Run &run, cx<T> *buf, int lp, const cx<T> *tw, int B) {
+  FPass<T, R, M, Ns, S> ps{buf, lp, tw, B * (M / R)};
   run(ps);
-  if constexpr (sizeof...(Rest) > 0) fft_seq<T, M, Ns * R, S, Run, Rest...>(run, buf, lp, tw + Ns * tw_stride(R));
+  if constexpr (sizeof...(Rest) > 0)
+    fft_seq<T, M, Ns * R, S, Run, Rest...>(run, buf, lp, tw + Ns * tw_stride(R), B);
 }
 
-// Radix sequence of each supported length (all radices divide P3E_FAST_E).
+// Radix sequence of each supported length (the radices of fast_radices in
+// pass_args.h; the odd radix of 3 or 5 times a power of two comes last).
 template <typename T, int M, int S, typename Run>
-P3E_HD P3E_INL void fft_pow2(Run &run, cx<T> *buf, int lp, const cx<T> *tw) {
-  if constexpr (M == 16) fft_seq<T, M, 1, S, Run, 16>(run, buf, lp, tw);
-  else if constexpr (M == 32) fft_seq<T, M, 1, S, Run, 8, 4>(run, buf, lp, tw);
-  else if constexpr (M == 64) fft_seq<T, M, 1, S, Run, 8, 8>(run, buf, lp, tw);
-  else if constexpr (M == 128) fft_seq<T, M, 1, S, Run, 16, 8>(run, buf, lp, tw);
-  else if constexpr (M == 256) fft_seq<T, M, 1, S, Run, 16, 16>(run, buf, lp, tw);
-  else if constexpr (M == 512) fft_seq<T, M, 1, S, Run, 8, 8, 8>(run, buf, lp, tw);
-  else if constexpr (M == 1024) fft_seq<T, M, 1, S, Run, 16, 8, 8>(run, buf, lp, tw);
-  else if constexpr (M == 2048) fft_seq<T, M, 1, S, Run, 16, 16, 8>(run, buf, lp, tw);
-  else fft_seq<T, M, 1, S, Run, 16, 16, 16>(run, buf, lp, tw);
+P3E_HD P3E_INL void fft_fixed(Run &run, cx<T> *buf, int lp, const cx<T> *tw, int B) {
+  if constexpr (M == 16) fft_seq<T, M, 1, S, Run, 16>(run, buf, lp, tw, B);
+  else if constexpr (M == 32) fft_seq<T, M, 1, S, Run, 8, 4>(run, buf, lp, tw, B);
+  else if constexpr (M == 64) fft_seq<T, M, 1, S, Run, 8, 8>(run, buf, lp, tw, B);
+  else if constexpr (M == 128) fft_seq<T, M, 1, S, Run, 16, 8>(run, buf, lp, tw, B);
+  else if constexpr (M == 256) fft_seq<T, M, 1, S, Run, 16, 16>(run, buf, lp, tw, B);
+  else if constexpr (M == 512) fft_seq<T, M, 1, S, Run, 8, 8, 8>(run, buf, lp, tw, B);
+  else if constexpr (M == 1024) fft_seq<T, M, 1, S, Run, 16, 8, 8>(run, buf, lp, tw, B);
+  else if constexpr (M == 2048) fft_seq<T, M, 1, S, Run, 16, 16, 8>(run, buf, lp, tw, B);
+  else if constexpr (M == 4096) fft_seq<T, M, 1, S, Run, 16, 16, 16>(run, buf, lp, tw, B);
+  else if constexpr (M == 48) fft_seq<T, M, 1, S, Run, 16, 3>(run, buf, lp, tw, B);
+  else if constexpr (M == 96) fft_seq<T, M, 1, S, Run, 8, 4, 3>(run, buf, lp, tw, B);
+  else if constexpr (M == 192) fft_seq<T, M, 1, S, Run, 16, 4, 3>(run, buf, lp, tw, B);
+  else if constexpr (M == 384) fft_seq<T, M, 1, S, Run, 16, 8, 3>(run, buf, lp, tw, B);
+  else if constexpr (M == 768) fft_seq<T, M, 1, S, Run, 16, 16, 3>(run, buf, lp, tw, B);
+  else if constexpr (M == 80) fft_seq<T, M, 1, S, Run, 16, 5>(run, buf, lp, tw, B);
+  else if constexpr (M == 160) fft_seq<T, M, 1, S, Run, 8, 4, 5>(run, buf, lp, tw, B);
+  else fft_seq<T, M, 1, S, Run, 16, 4, 5>(run, buf, lp, tw, B);  // 320
 }
 
 // Tile coordinates of workgroup `tile`.
@@ -139,17 +166,16 @@ template <typename T> struct TileRegs {
 // (line j, element n) of tile entry i in load order.
 template <int M>
 P3E_HD P3E_INL void fast_load_index(const PassArgs &p, int i, int *j, int *n) {
-  constexpr int logM = Log2<M>::v;
   if (p.load_order == ORD_LINE) {
     if (p.cls == CLS_R2C) {
-      *j = i >> (logM + 1);
-      *n = i & (2 * M - 1);
+      *j = qdiv<2 * M>(i);
+      *n = qmod<2 * M>(i);
     } else if (p.cls == CLS_C2R) {
       *j = (int)fdiv((uint32_t)i, p.div_nin);
       *n = i - *j * p.nin;
     } else {
-      *j = i >> logM;
-      *n = i & (M - 1);
+      *j = qdiv<M>(i);
+      *n = qmod<M>(i);
     }
   } else {
     *j = i & (p.B - 1);
@@ -208,7 +234,6 @@ template <typename V> P3E_HD P3E_INL void stg(char *base, uint32_t off, const V 
 template <typename T, int M, bool FULL>
 P3E_HD P3E_INL void fast_fetch(const PassArgs &p, uint32_t f, int e0, int nb, TileRegs<T> &R, int tid, int nth) {
   constexpr int E = P3E_FAST_E;
-  constexpr int logM = Log2<M>::v;
   constexpr uint32_t RS = sizeof(T), CS = 2 * sizeof(T);
   const uint32_t esz = p.cls == CLS_R2C ? RS : CS;
   const char *tin = reinterpret_cast<const char *>(p.in) + ((long long)e0 * p.is_e + (long long)f * p.is_f) * esz;
@@ -221,7 +246,7 @@ P3E_HD P3E_INL void fast_fetch(const PassArgs &p, uint32_t f, int e0, int nb, Ti
 #pragma unroll
     for (int q = 0; q < E / 2; q++) {
       const int i = tid + q * nth;
-      const int j = i >> (logM - 1), n = 2 * (i & (M / 2 - 1));
+      const int j = qdiv<M / 2>(i), n = 2 * qmod<M / 2>(i);
       const bool ok = FULL || j < nb;
       const CxPair<T> x = ldg<CxPair<T>>(tin, ok ? (uint32_t)(n + j * ise) * CS : 0u);
       R.v[4 * q] = x.ar;
@@ -249,7 +274,7 @@ P3E_HD P3E_INL void fast_fetch(const PassArgs &p, uint32_t f, int e0, int nb, Ti
 #pragma unroll
     for (int q = 0; q < E; q++) {
       const int i = tid + q * nth;
-      const int j = i >> logM, m = i & (M - 1);
+      const int j = qdiv<M>(i), m = qmod<M>(i);
       const bool ok = FULL || j < nb;
       const RealPair<T> x = ldg<RealPair<T>>(tin, ok ? (uint32_t)(2 * m + j * ise) * RS : 0u);
       R.v[2 * q] = x.a;
@@ -313,14 +338,13 @@ template <typename T, int M, bool FULL>
 P3E_HD P3E_INL void fast_put(const PassArgs &p, uint32_t f, int e0, int nb, const TileRegs<T> &R, cx<T> *buf,
                              int tid, int nth, const cx<T> *wN) {
   constexpr int E = P3E_FAST_E;
-  constexpr int logM = Log2<M>::v;
   const int tot = (p.load_order == ORD_LINE ? nb : p.B) * p.nin;
   const int lp = p.lp;
   if (p.vec2 && p.cls == CLS_C2C && p.load_order == ORD_LINE) {
 #pragma unroll
     for (int q = 0; q < E / 2; q++) {
       const int i = tid + q * nth;
-      const int j = i >> (logM - 1), n = 2 * (i & (M / 2 - 1));
+      const int j = qdiv<M / 2>(i), n = 2 * qmod<M / 2>(i);
       if (FULL || j < nb) {
         // n is even: n and n + 1 share a padding group
         cx<T> *d = buf + j * lp + P3E_PAD(n);
@@ -342,7 +366,7 @@ P3E_HD P3E_INL void fast_put(const PassArgs &p, uint32_t f, int e0, int nb, cons
 #pragma unroll
     for (int q = 0; q < E; q++) {
       const int i = tid + q * nth;
-      const int j = i >> logM, m = i & (M - 1);
+      const int j = qdiv<M>(i), m = qmod<M>(i);
       if (FULL || j < nb) buf[j * lp + P3E_PAD(m)] = mk(R.v[2 * q], R.v[2 * q + 1]);
     }
   } else if (p.vec2) {
@@ -426,8 +450,8 @@ P3E_HD P3E_INL cx<T> out_rule(const PassArgs &p, cx<T> y, int k, int je, uint32_
 //   X_k = (Z_k + conj Z_{M-k})/2 - i w_N^k (Z_k - conj Z_{M-k})/2
 template <typename T, int M>
 P3E_HD P3E_INL cx<T> r2c_split(const cx<T> *l, const cx<T> *wN, int k) {
-  const cx<T> zk = l[P3E_PAD(k & (M - 1))];
-  const cx<T> zm = l[P3E_PAD((M - k) & (M - 1))];
+  const cx<T> zk = l[P3E_PAD(qmod<M>(k))];
+  const cx<T> zm = l[P3E_PAD(qmod<M>(M - k))];
   const cx<T> e = add(zk, conj(zm)), o = sub(zk, conj(zm));
   return scl(add(e, rot(mul(wN[k], o), -1)), (T)0.5);
 }
@@ -461,7 +485,6 @@ P3E_HD P3E_INL void store_pair_tile(const PassArgs &p, uint32_t f, int e0, const
 template <typename T, int M, bool FULL>
 P3E_HD P3E_INL void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, const cx<T> *buf, int tid, int nth,
                                const cx<T> *wN) {
-  constexpr int logM = Log2<M>::v;
   constexpr int E = P3E_FAST_E;
   constexpr uint32_t RS = sizeof(T), CS = 2 * sizeof(T);
   const uint32_t esz = p.cls == CLS_C2R ? RS : CS;
@@ -475,7 +498,7 @@ P3E_HD P3E_INL void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, co
 #pragma unroll 4
     for (int q = 0; q < E; q++) {
       const int i = tid + q * nth;
-      const int j = i >> logM, m = i & (M - 1);
+      const int j = qdiv<M>(i), m = qmod<M>(i);
       if (!FULL && j >= nb) continue;
       const cx<T> z = buf[j * lp + P3E_PAD(m)];
       RealPair<T> v;
@@ -517,8 +540,8 @@ P3E_HD P3E_INL void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, co
       const int i = tid + q * nth;
       int k, j;
       if (p.store_order == ORD_LINE) {
-        j = i >> (logM + 1);
-        k = i & (2 * M - 1);
+        j = qdiv<2 * M>(i);
+        k = qmod<2 * M>(i);
       } else {
         j = i & (p.B - 1);
         k = i >> p.logB;
@@ -535,7 +558,7 @@ P3E_HD P3E_INL void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, co
 #pragma unroll
     for (int q = 0; q < E / 2; q++) {
       const int i = tid + q * nth;
-      const int j = i >> (logM - 1), k = 2 * (i & (M / 2 - 1));
+      const int j = qdiv<M / 2>(i), k = 2 * qmod<M / 2>(i);
       if (!FULL && j >= nb) continue;
       const cx<T> *l = buf + j * lp + P3E_PAD(k);
       const cx<T> y0 = out_rule(p, l[0], k, e0 + j, f), y1 = out_rule(p, l[1], k + 1, e0 + j, f);
@@ -563,8 +586,8 @@ P3E_HD P3E_INL void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, co
           j = (int)fdiv((uint32_t)i, p.div_nout);
           k = i - j * p.nout;
         } else {
-          j = i >> logM;
-          k = i & (M - 1);
+          j = qdiv<M>(i);
+          k = qmod<M>(i);
         }
         if (!FULL && j >= nb) continue;
       } else {
@@ -599,8 +622,8 @@ P3E_HD P3E_INL void fast_tile(const PassArgs &p, uint32_t tile, cx<T> *buf, int 
     if (full) fast_load<T, M, true>(p, f, e0, nb, buf, tid, nth, wN);
     else fast_load<T, M, false>(p, f, e0, nb, buf, tid, nth, wN);
   });
-  if (p.cls == CLS_R2C || p.sign < 0) fft_pow2<T, M, -1>(run, buf, p.lp, tw);
-  else fft_pow2<T, M, 1>(run, buf, p.lp, tw);
+  if (p.cls == CLS_R2C || p.sign < 0) fft_fixed<T, M, -1>(run, buf, p.lp, tw, p.B);
+  else fft_fixed<T, M, 1>(run, buf, p.lp, tw, p.B);
   each([&](int tid) P3E_LAMBDA_INL {
     if (tid >= nth) return;
     if (full) fast_store<T, M, true>(p, f, e0, nb, buf, tid, nth, wN);

@@ -1,64 +1,13 @@
-// Power-of-two pass kernels (fft_fast.h) and their launcher.
-#include <hip/hip_runtime.h>
-
-#include "fft_core.h"
-#include "fft_fast.h"
-#include "pass_dev.h"
+// Power-of-two pass kernels of the compile-time-length path and the
+// launchers of that path (other lengths: fft_pow2x.hip).
+#include "fixed_kernels.h"
 
 namespace p3e {
 
-// Power-of-two path: complex length M at compile time, one LDS buffer for
-// the tile, followed in LDS by the twiddle table (M entries) and, for
-// R2C/C2R, the split factors (M + 1 entries): the radix passes and the
-// split/merge read them from LDS instead of issuing global loads (which
-// would compete with the tile's stream for the memory pipeline).  The
-// twiddles are first read after the barrier that ends the load phase; the
-// split factors are read by the C2R load phase itself, so R2C/C2R passes
-// wait at a barrier after copying them.
-template <typename T, int M>
-__device__ __forceinline__ void pow2_body(const PassArgs &p, uint32_t tile) {
-  extern __shared__ unsigned char smem_raw[];
-  cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
-  cx<T> *ltw = buf + (size_t)p.B * p.lp;
-  cx<T> *lwn = ltw + M;
-  const cx<T> *gtw = reinterpret_cast<const cx<T> *>(p.tw);
-  const cx<T> *gwn = reinterpret_cast<const cx<T> *>(p.wN);
-  const int tid = threadIdx.x, nth = p.nthreads;
-  for (int i = tid; i < p.ntw; i += blockDim.x) ltw[i] = gtw[i];
-  if (p.cls != CLS_C2C) {
-    for (int i = tid; i <= M; i += blockDim.x) lwn[i] = gwn[i];
-    // the C2R load phase merges (X_k, X_{M-k}) with the split factors, so
-    // they must be in LDS before any thread's load phase
-    __syncthreads();
-  }
-  DevPassRun<T> run{tid, nth};
-  DevEach each{tid};
-  fast_tile<T, M>(p, tile, buf, nth, run, each, ltw, lwn);
-}
-
-template <typename T, int M>
-__global__ void __launch_bounds__(P3E_FAST_MAX_THREADS) pass_kernel_pow2(PassArgs p) {
-  pow2_body<T, M>(p, blockIdx.x);
-}
-
-// The same pass of many transforms in one launch (a batch stage): item i
-// owns workgroups start[i] .. start[i+1] - 1 and runs with tab[i]; the
-// workgroup size is the largest nthreads of the items (the others idle).
-template <typename T, int M>
-__global__ void __launch_bounds__(P3E_FAST_MAX_THREADS)
-    pass_kernel_pow2_grp(const PassArgs *__restrict__ tab, const uint32_t *__restrict__ start, int n) {
-  const uint32_t i = group_item(start, n, blockIdx.x);
-  pow2_body<T, M>(tab[i], blockIdx.x - start[i]);
-}
-
-template <typename T, int M>
-static int launch_pow2(const PassArgs *a, size_t lds_bytes, int threads, unsigned int nblocks, hipStream_t st) {
-  if (lds_bytes > 65536)
-    (void)hipFuncSetAttribute((const void *)pass_kernel_pow2<T, M>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lds_bytes);
-  hipLaunchKernelGGL((pass_kernel_pow2<T, M>), dim3(nblocks), dim3(threads), lds_bytes, st, *a);
-  return (int)hipGetLastError();
-}
+// fft_pow2x.hip: a pass (a) or a grouped launch (tab, start, n) of length
+// 3 or 5 times a power of two.
+int launch_fixed_odd(int M, const PassArgs *a, const PassArgs *tab, const uint32_t *start, int n, int esz,
+                     size_t lds_bytes, int threads, unsigned int nblocks, hipStream_t st);
 
 template <typename T>
 static int launch_pow2_any(const PassArgs *a, size_t lds_bytes, int threads, unsigned int nblocks, hipStream_t st) {
@@ -73,17 +22,7 @@ static int launch_pow2_any(const PassArgs *a, size_t lds_bytes, int threads, uns
     case 2048: return launch_pow2<T, 2048>(a, lds_bytes, threads, nblocks, st);
     case 4096: return launch_pow2<T, 4096>(a, lds_bytes, threads, nblocks, st);
   }
-  return (int)hipErrorInvalidValue;
-}
-
-template <typename T, int M>
-static int launch_pow2_grp(const PassArgs *tab, const uint32_t *start, int n, size_t lds_bytes, int threads,
-                           unsigned int nblocks, hipStream_t st) {
-  if (lds_bytes > 65536)
-    (void)hipFuncSetAttribute((const void *)pass_kernel_pow2_grp<T, M>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lds_bytes);
-  hipLaunchKernelGGL((pass_kernel_pow2_grp<T, M>), dim3(nblocks), dim3(threads), lds_bytes, st, tab, start, n);
-  return (int)hipGetLastError();
+  return launch_fixed_odd(a->M, a, nullptr, nullptr, 0, sizeof(T), lds_bytes, threads, nblocks, st);
 }
 
 template <typename T>
@@ -100,7 +39,7 @@ static int launch_pow2_grp_any(int M, const PassArgs *tab, const uint32_t *start
     case 2048: return launch_pow2_grp<T, 2048>(tab, start, n, lds_bytes, threads, nblocks, st);
     case 4096: return launch_pow2_grp<T, 4096>(tab, start, n, lds_bytes, threads, nblocks, st);
   }
-  return (int)hipErrorInvalidValue;
+  return launch_fixed_odd(M, nullptr, tab, start, n, sizeof(T), lds_bytes, threads, nblocks, st);
 }
 
 // A grouped launch of power-of-two passes of complex length M (tab and

@@ -49,15 +49,43 @@ struct HostEach {
   }
 };
 
+// The workgroup's LDS as the kernel lays it out (fixed_kernels.h): the tile,
+// then the twiddle table, then the R2C/C2R split factors, in exactly the
+// planned lds_bytes, followed by a guard region that must stay untouched.
+// Passes planned for the batch step kernel read their tables from global
+// memory (their LDS holds the tile only).
+static bool g_batch_plans = false;
+
 template <typename T, int M> static int run_pow2(const PassDesc &pd, const PassArgs &a) {
   HostPassRun<T> run{pd.threads, std::vector<cx<T>>((size_t)pd.threads * P3E_FAST_E)};
   HostEach each{pd.threads};
-  std::vector<cx<T>> buf((size_t)a.B * a.lp);
-  for (unsigned int t = 0; t < pd.nblocks; t++) {
-    for (auto &x : buf) x = cx<T>{(T)1e30, (T)-1e30};
-    fast_tile<T, M>(a, t, buf.data(), pd.threads, run, each, reinterpret_cast<const cx<T> *>(a.tw),
-                    reinterpret_cast<const cx<T> *>(a.wN));
+  const size_t n_lds = pd.lds_bytes / sizeof(cx<T>), guard = 64;
+  const size_t tile = (size_t)a.B * a.lp;
+  const size_t ntab = g_batch_plans ? 0 : (size_t)a.ntw + (a.cls == CLS_C2C ? 0 : M + 1);
+  if (tile + ntab > n_lds) {
+    set_error("power-of-two pass: tile and tables need %zu entries, LDS holds %zu",
+              tile + ntab, n_lds);
+    return -1;
   }
+  std::vector<cx<T>> lds(n_lds + guard);
+  const cx<T> poison{(T)1e30, (T)-1e30};
+  for (size_t i = n_lds; i < n_lds + guard; i++) lds[i] = poison;
+  const cx<T> *gtw = reinterpret_cast<const cx<T> *>(a.tw), *gwn = reinterpret_cast<const cx<T> *>(a.wN);
+  cx<T> *ltw = lds.data() + tile, *lwn = ltw + a.ntw;
+  if (!g_batch_plans) {
+    for (int i = 0; i < a.ntw; i++) ltw[i] = gtw[i];
+    if (a.cls != CLS_C2C)
+      for (int i = 0; i <= M; i++) lwn[i] = gwn[i];
+  }
+  for (unsigned int t = 0; t < pd.nblocks; t++) {
+    for (size_t i = 0; i < tile; i++) lds[i] = poison;
+    fast_tile<T, M>(a, t, lds.data(), pd.threads, run, each, g_batch_plans ? gtw : ltw, g_batch_plans ? gwn : lwn);
+  }
+  for (size_t i = n_lds; i < n_lds + guard; i++)
+    if (lds[i].x != poison.x || lds[i].y != poison.y) {
+      set_error("power-of-two pass wrote past its LDS");
+      return -1;
+    }
   return 0;
 }
 
@@ -115,6 +143,14 @@ template <typename T> static int run_pass(const PassDesc &pd, const void *in, vo
       case 1024: return run_pow2<T, 1024>(pd, a);
       case 2048: return run_pow2<T, 2048>(pd, a);
       case 4096: return run_pow2<T, 4096>(pd, a);
+      case 48: return run_pow2<T, 48>(pd, a);
+      case 96: return run_pow2<T, 96>(pd, a);
+      case 192: return run_pow2<T, 192>(pd, a);
+      case 384: return run_pow2<T, 384>(pd, a);
+      case 768: return run_pow2<T, 768>(pd, a);
+      case 80: return run_pow2<T, 80>(pd, a);
+      case 160: return run_pow2<T, 160>(pd, a);
+      case 320: return run_pow2<T, 320>(pd, a);
     }
     return -1;
   }
@@ -128,7 +164,7 @@ template <typename T> static int run_pass(const PassDesc &pd, const void *in, vo
   return 0;
 }
 
-static void run_chain(int prec, const PassChain &ch, const void *in, void *out, int idir) {
+static int run_chain(int prec, const PassChain &ch, const void *in, void *out, int idir) {
   std::vector<std::vector<char>> bufs(ch.passes.size());
   const void *src = in;
   for (size_t i = 0; i < ch.passes.size(); i++) {
@@ -140,10 +176,11 @@ static void run_chain(int prec, const PassChain &ch, const void *in, void *out, 
       dst = bufs[i].data();
     }
     const int deriv = (idir >= 0 && idir == pd.axis && pd.deriv_capable) ? 1 : 0;
-    if (prec == 8) run_pass<double>(pd, src, dst, deriv);
-    else run_pass<float>(pd, src, dst, deriv);
+    const int rc = prec == 8 ? run_pass<double>(pd, src, dst, deriv) : run_pass<float>(pd, src, dst, deriv);
+    if (rc) return rc;
     src = dst;
   }
+  return 0;
 }
 
 extern "C" {
@@ -177,8 +214,8 @@ int emu_exec(int prec, const int *cls, const int *sign, const int *kind, const i
     memcpy(out, in, out_bytes);
     return 0;
   }
-  run_chain(prec, ch, in, out, idir);
-  return 0;
+  g_batch_plans = batch_threads > 0;
+  return run_chain(prec, ch, in, out, idir);
 }
 
 // As emu_exec in batch mode, with every pass that accepts it re-tiled by
@@ -222,7 +259,8 @@ int emu_exec_retuned(int prec, const int *cls, const int *sign, const int *kind,
     strncpy(desc, d.c_str(), desc_len - 1);
     desc[desc_len - 1] = 0;
   }
-  run_chain(prec, ch, in, out, -1);
+  g_batch_plans = true;  // retuned passes are batch-kernel plans
+  if (run_chain(prec, ch, in, out, -1)) return -1;
   return n;
 }
 
@@ -236,8 +274,8 @@ int emu_poisson(int prec, const int *dims, const int *mo_real, const int *mo_spe
     strncpy(desc, (f.desc + " || " + b.desc).c_str(), desc_len - 1);
     desc[desc_len - 1] = 0;
   }
-  run_chain(prec, f, rhs, spec, -1);
-  run_chain(prec, b, spec, sol, -1);
+  g_batch_plans = batch_threads > 0;
+  if (run_chain(prec, f, rhs, spec, -1) || run_chain(prec, b, spec, sol, -1)) return -1;
   return 0;
 }
 

@@ -350,3 +350,34 @@ def test_padded_intermediate_arrays(prec, mo1, mo2):
     finally:
         emu.set_pad_threshold(32 << 20)
 
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+@pytest.mark.parametrize("M", [48, 96, 192, 384, 768, 80, 160, 320])
+def test_fixed_length_odd_radix(M, prec):
+    """Compile-time lengths 3 and 5 times a power of two (the odd radix pass
+    has ceil(16 / R) butterfly slots per thread): C2C along a stride-1 axis
+    (line order) and a strided axis (tile order, a partial last tile), R2C
+    of length 2M and its C2R, against the oracle."""
+    s = "D" if prec == 8 else "S"
+    cf = C.CF_D if prec == 8 else C.CF_S
+    cases = [([M, 3, 2], [0, 1, 2], [0, 1, 2]), ([5, M, 3], [0, 1, 2], [2, 0, 1])]
+    for dims, mo1, mo2 in cases:
+        x = (np.random.default_rng(M).standard_normal(orc.storage_shape(dims, mo1)) +
+             1j * np.random.default_rng(M + 1).standard_normal(orc.storage_shape(dims, mo1)))
+        got, desc = emu.exec3d([cf, cf, cf], dims, mo1, dims, mo2, x, prec)
+        assert f"N={M}, pow2" in desc, desc
+        ref = orc.exec3d([C.CF_D] * 3, dims, mo1, mo2, x)
+        assert C.rel_l2(got, ref) < TOL[prec], desc
+    r2c = C.R2C_D if prec == 8 else C.R2C_S
+    c2r = C.C2R_D if prec == 8 else C.C2R_S
+    for dims, mo1, mo2 in (([2 * M, 3, 2], [0, 1, 2], [0, 1, 2]), ([2 * M, 6, 3], [2, 1, 0], [2, 1, 0])):
+        x = np.random.default_rng(2 * M).uniform(-1, 1, orc.storage_shape(dims, mo1))
+        od = [M + 1, dims[1], dims[2]]
+        spec, desc = emu.exec3d([r2c, cf, cf], dims, mo1, od, mo2, x, prec)
+        assert f"r2c[axis 0, N={2 * M}, pow2" in desc, desc
+        ref = orc.exec3d([C.R2C_D, C.CF_D, C.CF_D], dims, mo1, mo2, x)
+        assert C.rel_l2(spec, ref) < TOL[prec], desc
+        cb = C.CB_D if prec == 8 else C.CB_S
+        back, desc_b = emu.exec3d([c2r, cb, cb], od, mo2, dims, mo1, spec, prec)
+        assert C.rel_l2(back / np.prod(dims), x) < (1e-13 if prec == 8 else 1e-6), desc_b
