@@ -283,9 +283,10 @@ static void setup_mid(int prec, const OpSpec &op, PassDesc &ps, int batch_thread
   ps.nblocks = (unsigned int)((unsigned long long)a.ntile_e * (unsigned long long)a.nf);
 }
 
-// Switch an already described pass to the power-of-two path when it is a
-// C2C, R2C or C2R of power-of-two complex length M in [16, 4096] (M = N/2 for
-// the real transforms).  Tile: B lines, B as large as the LDS budget and
+// Switch an already described pass to the compile-time-length path when it
+// is a C2C, R2C, C2R or complex DCT-I whose complex length M is one of the
+// fast lengths (fast_radices: powers of two 16..4096, 3 and 5 times a power
+// of two; M = N/2 for the real transforms, 2 (N - 1) for DCT-I).  Tile: B lines, B as large as the LDS budget and
 // P3E_FAST_MAX_THREADS threads allow (up to 64), P3E_FAST_E elements per
 // thread.
 static void setup_fast(int prec, const OpSpec &op, PassDesc &ps, int batch_threads) {
@@ -294,7 +295,10 @@ static void setup_fast(int prec, const OpSpec &op, PassDesc &ps, int batch_threa
   int M;
   if (op.cls == CLS_C2C) M = a.N;
   else if ((op.cls == CLS_R2C || op.cls == CLS_C2R) && a.N % 2 == 0) M = a.N / 2;
+  // DCT-I of complex lines: the M = 2 (N - 1)-point FFT of the even extension
+  else if (op.cls == CLS_R2R && op.r2r_kind == REDFT00 && a.in_cplx && a.out_cplx) M = a.W;
   else return;
+  const bool split = op.cls == CLS_R2C || op.cls == CLS_C2R;
   int rad[P3E_MAX_RADIX_PASSES];
   const int nrad = fast_radices(M, rad);
   if (nrad == 0) return;
@@ -332,7 +336,7 @@ static void setup_fast(int prec, const OpSpec &op, PassDesc &ps, int batch_threa
       ns *= rad[i];
     }
   }
-  const size_t tables = batch_threads > 0 ? 0 : (ntw + (op.cls == CLS_C2C ? 0 : M + 1)) * esz;
+  const size_t tables = batch_threads > 0 ? 0 : (ntw + (split ? M + 1 : 0)) * esz;
   auto lds = [&](int b) { return (size_t)b * lp * esz + tables; };
   // (a 160 KiB budget, B = 64 lines of 256 or 32 of 512 with one workgroup
   // per CU, measured slower on MI355X: 0.31 ms vs 0.27 ms for the R2C pass
@@ -351,11 +355,11 @@ static void setup_fast(int prec, const OpSpec &op, PassDesc &ps, int batch_threa
   a.mid = 0;
   a.mid_prime = 0;
   a.pair = 0;
-  // R2C or C2C with a tile-order load along a stride-1 e: two adjacent
+  // R2C, C2C or DCT-I with a tile-order load along a stride-1 e: two adjacent
   // lines' elements per load when every tile is whole and the strides keep
   // pairs aligned (the launcher also checks the array's address);
   // standalone passes only
-  a.vec2 = batch_threads == 0 && (op.cls == CLS_R2C || op.cls == CLS_C2C) && a.load_order == ORD_TILE &&
+  a.vec2 = batch_threads == 0 && op.cls != CLS_C2R && a.load_order == ORD_TILE &&
            a.is_e == 1 && B >= 2 && a.ne % B == 0 && a.is_d % 2 == 0 && a.is_f % 2 == 0;
   // and the mirror for all three with a tile-order store along a stride-1 e
   a.vec2s = batch_threads == 0 && a.store_order == ORD_TILE && a.os_e == 1 && B >= 2 && a.ne % B == 0 &&
@@ -398,7 +402,7 @@ static void setup_fast(int prec, const OpSpec &op, PassDesc &ps, int batch_threa
   pass_twiddles(rad, nrad, ps.tw);
   a.ntw = (int)ps.tw.size();  // ntw entries; the kernel puts the split factors after them
   ps.wN.clear();
-  if (op.cls != CLS_C2C) {
+  if (split) {
     ps.wN.resize(M + 1);
     for (int k = 0; k <= M; k++) ps.wN[k] = expi_pi(-2LL * k, a.N);
   }

@@ -170,7 +170,7 @@ P3E_HD P3E_INL void fast_load_index(const PassArgs &p, int i, int *j, int *n) {
     if (p.cls == CLS_R2C) {
       *j = qdiv<2 * M>(i);
       *n = qmod<2 * M>(i);
-    } else if (p.cls == CLS_C2R) {
+    } else if (p.cls == CLS_C2R || p.cls == CLS_R2R) {
       *j = (int)fdiv((uint32_t)i, p.div_nin);
       *n = i - *j * p.nin;
     } else {
@@ -241,7 +241,36 @@ P3E_HD P3E_INL void fast_fetch(const PassArgs &p, uint32_t f, int e0, int nb, Ti
   const int tot = (p.load_order == ORD_LINE ? nb : p.B) * p.nin;
   // out-of-tile entries load the tile's first element (always valid) and
   // are not written: unconditional loads keep the values in registers
-  if (p.vec2 && p.cls == CLS_C2C && p.load_order == ORD_LINE) {
+  if (p.cls == CLS_R2R) {
+    // DCT-I (REDFT00) of complex lines of N = M/2 + 1 points: B N = B M/2 + B
+    // entries per tile, covered by E/2 + 1 entries per thread (nth >= B);
+    // paired (vec2: tile order, e stride-1, whole tiles), element n of lines
+    // j = 2 jp and j + 1 per load, B N / 2 pairs in E/4 + 1 entries
+    if (p.vec2) {
+      const int lb = p.logB - 1, ptot = (p.B >> 1) * p.nin;
+#pragma unroll
+      for (int q = 0; q < E / 4 + 1; q++) {
+        const int i = tid + q * nth;
+        const int j = 2 * (i & ((1 << lb) - 1)), n = i >> lb;
+        const CxPair<T> x = ldg<CxPair<T>>(tin, i < ptot ? (uint32_t)(n * isd + j) * CS : 0u);
+        R.v[4 * q] = x.ar;
+        R.v[4 * q + 1] = x.ai;
+        R.v[4 * q + 2] = x.br;
+        R.v[4 * q + 3] = x.bi;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < E / 2 + 1; q++) {
+        const int i = tid + q * nth;
+        int j, n;
+        fast_load_index<M>(p, i, &j, &n);
+        const bool ok = i < tot && (FULL || j < nb);
+        const cx<T> x = ldg<cx<T>>(tin, ok ? (uint32_t)(n * isd + j * ise) * CS : 0u);
+        R.v[2 * q] = x.x;
+        R.v[2 * q + 1] = x.y;
+      }
+    }
+  } else if (p.vec2 && p.cls == CLS_C2C && p.load_order == ORD_LINE) {
     // C2C, line order (d stride-1): elements n = 2 np and n + 1 of line j
 #pragma unroll
     for (int q = 0; q < E / 2; q++) {
@@ -340,7 +369,38 @@ P3E_HD P3E_INL void fast_put(const PassArgs &p, uint32_t f, int e0, int nb, cons
   constexpr int E = P3E_FAST_E;
   const int tot = (p.load_order == ORD_LINE ? nb : p.B) * p.nin;
   const int lp = p.lp;
-  if (p.vec2 && p.cls == CLS_C2C && p.load_order == ORD_LINE) {
+  if (p.cls == CLS_R2R) {
+    // even extension: element n at positions n and M - n (0 < n < M/2)
+    if (p.vec2) {
+      const int lb = p.logB - 1, ptot = (p.B >> 1) * p.nin;
+#pragma unroll
+      for (int q = 0; q < E / 4 + 1; q++) {
+        const int i = tid + q * nth;
+        if (i >= ptot) continue;
+        const int j = 2 * (i & ((1 << lb) - 1)), n = i >> lb;
+        const cx<T> a = mk(R.v[4 * q], R.v[4 * q + 1]), b = mk(R.v[4 * q + 2], R.v[4 * q + 3]);
+        cx<T> *d = buf + j * lp;
+        d[P3E_PAD(n)] = a;
+        d[lp + P3E_PAD(n)] = b;
+        if (n >= 1 && n < M / 2) {
+          d[P3E_PAD(M - n)] = a;
+          d[lp + P3E_PAD(M - n)] = b;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < E / 2 + 1; q++) {
+        const int i = tid + q * nth;
+        int j, n;
+        fast_load_index<M>(p, i, &j, &n);
+        if (i < tot && (FULL || j < nb)) {
+          const cx<T> x = mk(R.v[2 * q], R.v[2 * q + 1]);
+          buf[j * lp + P3E_PAD(n)] = x;
+          if (n >= 1 && n < M / 2) buf[j * lp + P3E_PAD(M - n)] = x;
+        }
+      }
+    }
+  } else if (p.vec2 && p.cls == CLS_C2C && p.load_order == ORD_LINE) {
 #pragma unroll
     for (int q = 0; q < E / 2; q++) {
       const int i = tid + q * nth;
@@ -574,15 +634,24 @@ P3E_HD P3E_INL void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, co
     // j = 2 jp and j + 1, one store.  (E/2) nth entries are exactly the C2C
     // tile (nout = M); an R2C tile has nout = M + 1, B/2 entries more.
     const int ptot = (p.B >> 1) * p.nout;
+    if (p.cls == CLS_R2R) {
+      // DCT-I: nout = M/2 + 1, B nout / 2 pairs in E/4 + 1 entries
 #pragma unroll
-    for (int q = 0; q < E / 2; q++) store_pair_tile<T, M>(p, f, e0, buf, wN, tout, tid + q * nth);
-    for (int i = tid + (E / 2) * nth; i < ptot; i += nth) store_pair_tile<T, M>(p, f, e0, buf, wN, tout, i);
+      for (int q = 0; q < E / 4 + 1; q++) {
+        const int i = tid + q * nth;
+        if (i < ptot) store_pair_tile<T, M>(p, f, e0, buf, wN, tout, i);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < E / 2; q++) store_pair_tile<T, M>(p, f, e0, buf, wN, tout, tid + q * nth);
+      for (int i = tid + (E / 2) * nth; i < ptot; i += nth) store_pair_tile<T, M>(p, f, e0, buf, wN, tout, i);
+    }
   } else {
 #pragma unroll 4
     for (int i = tid; i < tot; i += nth) {
       int k, j;
       if (p.store_order == ORD_LINE) {
-        if (p.cls == CLS_R2C) {
+        if (p.cls == CLS_R2C || p.cls == CLS_R2R) {
           j = (int)fdiv((uint32_t)i, p.div_nout);
           k = i - j * p.nout;
         } else {

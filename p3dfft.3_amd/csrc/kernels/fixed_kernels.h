@@ -28,7 +28,7 @@ __device__ __forceinline__ void pow2_body(const PassArgs &p, uint32_t tile) {
   const cx<T> *gwn = reinterpret_cast<const cx<T> *>(p.wN);
   const int tid = threadIdx.x, nth = p.nthreads;
   for (int i = tid; i < p.ntw; i += blockDim.x) ltw[i] = gtw[i];
-  if (p.cls != CLS_C2C) {
+  if (p.cls == CLS_R2C || p.cls == CLS_C2R) {
     for (int i = tid; i <= M; i += blockDim.x) lwn[i] = gwn[i];
     // the C2R load phase merges (X_k, X_{M-k}) with the split factors, so
     // they must be in LDS before any thread's load phase

@@ -61,7 +61,7 @@ template <typename T, int M> static int run_pow2(const PassDesc &pd, const PassA
   HostEach each{pd.threads};
   const size_t n_lds = pd.lds_bytes / sizeof(cx<T>), guard = 64;
   const size_t tile = (size_t)a.B * a.lp;
-  const size_t ntab = g_batch_plans ? 0 : (size_t)a.ntw + (a.cls == CLS_C2C ? 0 : M + 1);
+  const size_t ntab = g_batch_plans ? 0 : (size_t)a.ntw + (a.cls == CLS_R2C || a.cls == CLS_C2R ? M + 1 : 0);
   if (tile + ntab > n_lds) {
     set_error("power-of-two pass: tile and tables need %zu entries, LDS holds %zu",
               tile + ntab, n_lds);
@@ -74,7 +74,7 @@ template <typename T, int M> static int run_pow2(const PassDesc &pd, const PassA
   cx<T> *ltw = lds.data() + tile, *lwn = ltw + a.ntw;
   if (!g_batch_plans) {
     for (int i = 0; i < a.ntw; i++) ltw[i] = gtw[i];
-    if (a.cls != CLS_C2C)
+    if (a.cls == CLS_R2C || a.cls == CLS_C2R)
       for (int i = 0; i <= M; i++) lwn[i] = gwn[i];
   }
   for (unsigned int t = 0; t < pd.nblocks; t++) {

@@ -381,3 +381,28 @@ def test_fixed_length_odd_radix(M, prec):
         cb = C.CB_D if prec == 8 else C.CB_S
         back, desc_b = emu.exec3d([c2r, cb, cb], od, mo2, dims, mo1, spec, prec)
         assert C.rel_l2(back / np.prod(dims), x) < (1e-13 if prec == 8 else 1e-6), desc_b
+
+
+@pytest.mark.parametrize("n", [9, 17, 49, 129, 513])
+@pytest.mark.parametrize("prec", [4, 8])
+def test_fixed_length_dct1_complex(n, prec):
+    """Complex DCT-I of n points on the compile-time-length path (the
+    2 (n - 1)-point FFT of the even extension): line order (d stride-1),
+    tile order with paired lines (e stride-1, whole tiles) and tile order
+    with a partial last tile, against the oracle."""
+    tid = C.r2r_id("DCT1", True, prec)
+    cases = [([n, 6, 3], [0, 1, 2], [0, 1, 2]),   # line order both sides
+             ([16, 5, n], [0, 1, 2], [0, 1, 2]),  # tile order, e stride-1, whole tiles
+             ([7, 3, n], [0, 1, 2], [2, 0, 1]),   # tile load (partial tile), line store
+             ([n, 5, 4], [1, 2, 0], [0, 2, 1])]   # tile load, line store
+    rng = np.random.default_rng(n)
+    for dims, mo1, mo2 in cases:
+        shp = orc.storage_shape(dims, mo1)
+        x = rng.standard_normal(shp) + 1j * rng.standard_normal(shp)
+        ax = dims.index(n)
+        empty = C.EMPTY_DC if prec == 8 else C.EMPTY_SC
+        got, desc = emu.exec3d([empty if a != ax else tid for a in range(3)], dims, mo1, dims, mo2, x, prec)
+        if n - 1 >= 8:
+            assert f"r2r[axis {ax}, N={n}, pow2" in desc, desc
+        ref = orc.exec_1d(C.r2r_id("DCT1", True, 8), ax, dims, mo1, dims, mo2, x)
+        assert C.rel_l2(got, ref) < TOL[prec], desc
