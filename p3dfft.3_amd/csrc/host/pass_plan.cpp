@@ -380,6 +380,10 @@ static void setup_fast(int prec, const OpSpec &op, PassDesc &ps, int batch_threa
   if (batch_threads == 0 && op.cls == CLS_C2C && a.store_order == ORD_LINE && a.os_d == 1 && a.os_e % 2 == 0 &&
       a.os_f % 2 == 0)
     a.vec2s = 1;
+  // rows shorter than a cache line: adjacent tiles share lines, so they
+  // run on the same XCD (one L2); measured 0.81 -> 0.66 ms for the complex
+  // 513-point DCT-I pass (B = 8, 64-byte rows) of a 385x640x513 array
+  a.xcd_map = (a.load_order == ORD_TILE && B * in_esz < 128) || (a.store_order == ORD_TILE && B * out_esz < 128);
   a.tw_in_lds = 0;
   a.M = M;
   a.logM = 0;

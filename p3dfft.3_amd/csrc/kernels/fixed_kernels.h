@@ -41,7 +41,7 @@ __device__ __forceinline__ void pow2_body(const PassArgs &p, uint32_t tile) {
 
 template <typename T, int M>
 __global__ void __launch_bounds__(P3E_FAST_MAX_THREADS) pass_kernel_pow2(PassArgs p) {
-  pow2_body<T, M>(p, blockIdx.x);
+  pow2_body<T, M>(p, p.xcd_map ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x);
 }
 
 // The same pass of many transforms in one launch (a batch stage): item i

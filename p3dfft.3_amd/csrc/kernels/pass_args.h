@@ -93,6 +93,10 @@ struct PassArgs {
   // i - kn[r], with kn[r] the axis' logical real length.
   int spec_op;
   int kn[3], khalf[3];
+  // compile-time-length path: workgroups take tiles in the XCD-grouped
+  // order (pass_dev.h, xcd_tile), for tiles whose tile-order rows are
+  // shorter than a 128-byte cache line
+  int xcd_map;
 };
 
 // One entry of a batch work list: tile `tile` of the pass table[pass].
