@@ -137,7 +137,8 @@ P3E_HD P3E_INL void fft_fixed(Run &run, cx<T> *buf, int lp, const cx<T> *tw, int
   else if constexpr (M == 768) fft_seq<T, M, 1, S, Run, 16, 16, 3>(run, buf, lp, tw, B);
   else if constexpr (M == 80) fft_seq<T, M, 1, S, Run, 16, 5>(run, buf, lp, tw, B);
   else if constexpr (M == 160) fft_seq<T, M, 1, S, Run, 8, 4, 5>(run, buf, lp, tw, B);
-  else fft_seq<T, M, 1, S, Run, 16, 4, 5>(run, buf, lp, tw, B);  // 320
+  else if constexpr (M == 320) fft_seq<T, M, 1, S, Run, 16, 4, 5>(run, buf, lp, tw, B);
+  else fft_seq<T, M, 1, S, Run, 16, 8, 5>(run, buf, lp, tw, B);  // 640
 }
 
 // Tile coordinates of workgroup `tile`.
