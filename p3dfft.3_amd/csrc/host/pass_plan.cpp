@@ -342,6 +342,10 @@ static void setup_fast(int prec, const OpSpec &op, PassDesc &ps, int batch_threa
   // per CU, measured slower on MI355X: 0.31 ms vs 0.27 ms for the R2C pass
   // of a 512^3 grid, 0.30 ms vs 0.23 ms for a C2C pass)
   while (B * 2 <= Bmax && lds(B * 2) <= LDS_BUDGET && (long)M * B * 2 / P3E_FAST_E <= maxth) B *= 2;
+  // B M elements, P3E_FAST_E per thread: an exact number of threads (120 =
+  // 7.5 x 16 needs B even)
+  while ((long)M * B % P3E_FAST_E != 0 && B * 2 <= Bmax) B *= 2;
+  if ((long)M * B % P3E_FAST_E != 0) return;
   if (lds(B) > LDS_MAX || (long)M * B / P3E_FAST_E > maxth) return;
   // the kernel addresses a tile's elements with 32-bit byte offsets from
   // the tile's first element: the largest offset within a tile (one element

@@ -118,7 +118,7 @@ P3E_HD P3E_INL void fft_seq(Run &run, cx<T> *buf, int lp, const cx<T> *tw, int B
 }
 
 // Radix sequence of each supported length (the radices of fast_radices in
-// pass_args.h; the odd radix of 3 or 5 times a power of two comes last).
+// pass_args.h; odd radices come last).
 template <typename T, int M, int S, typename Run>
 P3E_HD P3E_INL void fft_fixed(Run &run, cx<T> *buf, int lp, const cx<T> *tw, int B) {
   if constexpr (M == 16) fft_seq<T, M, 1, S, Run, 16>(run, buf, lp, tw, B);
@@ -138,6 +138,7 @@ P3E_HD P3E_INL void fft_fixed(Run &run, cx<T> *buf, int lp, const cx<T> *tw, int
   else if constexpr (M == 80) fft_seq<T, M, 1, S, Run, 16, 5>(run, buf, lp, tw, B);
   else if constexpr (M == 160) fft_seq<T, M, 1, S, Run, 8, 4, 5>(run, buf, lp, tw, B);
   else if constexpr (M == 320) fft_seq<T, M, 1, S, Run, 16, 4, 5>(run, buf, lp, tw, B);
+  else if constexpr (M == 120) fft_seq<T, M, 1, S, Run, 8, 5, 3>(run, buf, lp, tw, B);
   else fft_seq<T, M, 1, S, Run, 16, 8, 5>(run, buf, lp, tw, B);  // 640
 }
 

@@ -1,5 +1,5 @@
 // Kernels of the compile-time-length path for 3 and 5 times a power of two
-// (48 ... 768, 80 ... 640) and their launcher.
+// (48 ... 768, 80 ... 640, 120) and their launcher.
 #include "fixed_kernels.h"
 
 namespace p3e {
@@ -21,6 +21,7 @@ static int launch_odd(int M, const PassArgs *a, const PassArgs *tab, const uint3
     P3E_ODD_CASE(160)
     P3E_ODD_CASE(320)
     P3E_ODD_CASE(640)
+    P3E_ODD_CASE(120)
   }
 #undef P3E_ODD_CASE
   return (int)hipErrorInvalidValue;

@@ -139,8 +139,8 @@ static inline void drop_unaligned_pairs(PassArgs &a, int prec) {
 
 // Radix sequence of the compile-time-length path for each complex length M
 // (the kernel's compile-time sequences in fft_fast.h are the same); 0 if M
-// is not supported.  Powers of two from 16 to 4096; 3 times 16 to 256 and 5
-// times 16 to 128, with the odd radix last.
+// is not supported.  Powers of two from 16 to 4096; 3 times 16 to 256, 5
+// times 16 to 128 and 120 = 8 5 3, with the odd radices last.
 static inline int fast_radices(int M, int *rad) {
   switch (M) {
     case 48: rad[0] = 16; rad[1] = 3; return 2;
@@ -152,6 +152,7 @@ static inline int fast_radices(int M, int *rad) {
     case 160: rad[0] = 8; rad[1] = 4; rad[2] = 5; return 3;
     case 320: rad[0] = 16; rad[1] = 4; rad[2] = 5; return 3;
     case 640: rad[0] = 16; rad[1] = 8; rad[2] = 5; return 3;
+    case 120: rad[0] = 8; rad[1] = 5; rad[2] = 3; return 3;
 
     case 16: rad[0] = 16; return 1;
     case 32: rad[0] = 8; rad[1] = 4; return 2;

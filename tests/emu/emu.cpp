@@ -152,6 +152,7 @@ template <typename T> static int run_pass(const PassDesc &pd, const void *in, vo
       case 160: return run_pow2<T, 160>(pd, a);
       case 320: return run_pow2<T, 320>(pd, a);
       case 640: return run_pow2<T, 640>(pd, a);
+      case 120: return run_pow2<T, 120>(pd, a);
     }
     return -1;
   }
