@@ -445,6 +445,25 @@ BatchPlan *batch_create(int prec, int nitems, const OpSpec ops[3], const int ord
     batch_destroy(b);
     return nullptr;
   }
+  if (b->grouped) {
+    size_t most = 0;
+    for (const auto &gs : b->groups) most = std::max(most, gs.size());
+    b->fork_ev.assign(b->nstages, nullptr);
+    for (int s = 0; s < b->nstages; s++) (void)hipEventCreateWithFlags(&b->fork_ev[s], hipEventDisableTiming);
+    for (size_t q = 0; q + 1 < most; q++) {
+      hipStream_t hs = nullptr;
+      hipEvent_t he = nullptr;
+      if (hipStreamCreateWithFlags(&hs, hipStreamNonBlocking) != hipSuccess ||
+          hipEventCreateWithFlags(&he, hipEventDisableTiming) != hipSuccess) {
+        if (hs) (void)hipStreamDestroy(hs);
+        set_error("batch: creation of a side stream failed");
+        batch_destroy(b);
+        return nullptr;
+      }
+      b->side.push_back(hs);
+      b->join_ev.push_back(he);
+    }
+  }
   const size_t ntab = b->table.size();
   if (hipHostMalloc((void **)&b->h_table[0], sizeof(PassArgs) * ntab, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void **)&b->h_table[1], sizeof(PassArgs) * ntab, hipHostMallocDefault) != hipSuccess ||
@@ -461,13 +480,30 @@ BatchPlan *batch_create(int prec, int nitems, const OpSpec ops[3], const int ord
 static int batch_launch(BatchPlan *b, hipStream_t st) {
   for (int s = 0; s < b->nstages; s++) {
     if (b->grouped) {
-      for (const BatchGroup &g : b->groups[s]) {
+      const std::vector<BatchGroup> &gs = b->groups[s];
+      // the groups of a stage are independent: run concurrently, one
+      // group's tail overlaps the others (1.99 -> 1.80 ms for 256 items of
+      // 32..128 points, six groups a stage)
+      const int nside = std::min((int)gs.size() - 1, (int)b->side.size());
+      if (nside > 0) {
+        BCHK(hipEventRecord(b->fork_ev[s], st));
+        for (int q = 0; q < nside; q++) BCHK(hipStreamWaitEvent(b->side[q], b->fork_ev[s], 0));
+      }
+      for (size_t gi = 0; gi < gs.size(); gi++) {
+        const BatchGroup &g = gs[gi];
+        // group gi on side stream gi - 1 (the exec stream takes group 0 and
+        // any beyond the side streams)
+        hipStream_t gst = (gi >= 1 && (int)gi <= nside) ? b->side[gi - 1] : st;
         const int rc = p3e_launch_group(g.kind, g.M, b->d_table + g.tab0, g.d_start, g.n, b->prec, g.lds, g.threads,
-                                        g.nblocks, st);
+                                        g.nblocks, gst);
         if (rc) {
           set_error("batch group launch failed: %s", hipGetErrorString((hipError_t)rc));
           return -1;
         }
+      }
+      for (int q = 0; q < nside; q++) {
+        BCHK(hipEventRecord(b->join_ev[q], b->side[q]));
+        BCHK(hipStreamWaitEvent(st, b->join_ev[q], 0));
       }
     } else {
       size_t lds = 0;
@@ -562,6 +598,10 @@ void batch_destroy(BatchPlan *b) {
     if (b->h_table[k]) (void)hipHostFree(b->h_table[k]);
     if (b->table_done[k]) (void)hipEventDestroy(b->table_done[k]);
   }
+  for (hipStream_t hs : b->side) (void)hipStreamDestroy(hs);
+  for (hipEvent_t he : b->join_ev) (void)hipEventDestroy(he);
+  for (hipEvent_t he : b->fork_ev)
+    if (he) (void)hipEventDestroy(he);
   for (void *p : b->owned) (void)hipFree(p);
   for (WorkItem *w : b->d_work)
     if (w) (void)hipFree(w);

@@ -54,6 +54,11 @@ struct BatchPlan {
   hipEvent_t table_done[2];  // the upload from staging copy k has been made
   bool table_pending[2];
   int parity;                // staging copy of the next exec
+  // grouped: the launches of one stage run on the exec stream and side
+  // streams, forked from and joined back into the exec stream by events
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> fork_ev;  // per stage
+  std::vector<hipEvent_t> join_ev;  // per side stream
   std::string desc;
 };
 
