@@ -201,6 +201,8 @@ static const size_t LDS_MAX = 160 * 1024;
 
 
 static bool g_fast_enabled = true;
+static bool g_dct_split = true;
+void set_dct_split(bool on) { g_dct_split = on; }
 void set_fast_path(bool enable) { g_fast_enabled = enable; }
 
 static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
@@ -283,6 +285,71 @@ static void setup_mid(int prec, const OpSpec &op, PassDesc &ps, int batch_thread
   ps.nblocks = (unsigned int)((unsigned long long)a.ntile_e * (unsigned long long)a.nf);
 }
 
+// Complex DCT-I of P + 1 points on the split form of the compile-time-length
+// path (fft_fast.h, dct1_tile): a P-point FFT and a P/2-point inverse FFT
+// per line instead of the 2P-point FFT of the even extension.  Tile: B
+// lines, (P + P/2) B / P3E_FAST_E threads, LDS for both line sets and the
+// tables.  Returns false (the pass unchanged) when P has no split form or
+// the tile does not fit.
+static bool setup_dct1_split(int prec, PassDesc &ps) {
+  PassArgs &a = ps.args;
+  const int P = a.N - 1, L = P / 2;
+  int ra[P3E_MAX_RADIX_PASSES], rb[P3E_MAX_RADIX_PASSES];
+  const int nrad = dct1_radices(P, ra, rb);
+  if (nrad == 0) return false;
+  const size_t esz = 2 * (size_t)prec;
+  const int lp = ((P + P / 16 + 31) & ~31) + 1, lp2 = ((L + L / 16 + 31) & ~31) + 1;
+  std::vector<cld> twa, twb;
+  pass_twiddles(ra, nrad, twa);
+  pass_twiddles(rb, nrad, twb);
+  const size_t ntw = twa.size() + twb.size() + (size_t)(L / 2 + 1);
+  auto lds = [&](int b) { return (size_t)b * (lp + lp2) * esz + ntw * esz; };
+  auto threads = [&](int b) { return (long)(P + L) * b / P3E_FAST_E; };
+  const int Bmax = std::min(64, next_pow2(a.ne));
+  int B = 1;
+  while (B * 2 <= Bmax && lds(B * 2) <= LDS_BUDGET && threads(B * 2) <= P3E_FAST_MAX_THREADS) B *= 2;
+  if (lds(B) > LDS_MAX || threads(B) > P3E_FAST_MAX_THREADS || (long)L * B % P3E_FAST_E != 0) return false;
+  const double span_in = ((double)a.nin * a.is_d + (double)B * a.is_e + 1) * esz;
+  const double span_out = ((double)a.nout * a.os_d + (double)B * a.os_e + 1) * esz;
+  if (span_in >= 2147483648.0 || span_out >= 2147483648.0) return false;
+  a.fast = 1;
+  a.dct_split = 1;
+  a.mid = 0;
+  a.mid_prime = 0;
+  a.pair = 0;
+  a.vec2 = 0;
+  a.vec2s = 0;
+  a.xcd_map = (a.load_order == ORD_TILE && B * esz < 128) || (a.store_order == ORD_TILE && B * esz < 128);
+  a.tw_in_lds = 0;
+  a.M = P;
+  a.logM = 0;
+  while ((1 << a.logM) < P) a.logM++;
+  a.lp = lp;
+  a.lp2 = lp2;
+  a.nrad = nrad;
+  for (int i = 0; i < nrad; i++) a.radix[i] = ra[i];
+  a.L = P;
+  a.zero_fill = 0;
+  a.B = B;
+  a.logB = 0;
+  while ((1 << a.logB) < B) a.logB++;
+  a.ntile_e = (a.ne + B - 1) / B;
+  a.div_ntile = make_fastdiv((uint32_t)a.ntile_e);
+  a.tab_off = B * (lp + lp2);
+  ps.lds_bytes = lds(B);
+  ps.threads = (int)threads(B);
+  a.nthreads = ps.threads;
+  ps.nblocks = (unsigned int)((unsigned long long)a.ntile_e * (unsigned long long)a.nf);
+  ps.tw = twa;
+  ps.tw.insert(ps.tw.end(), twb.begin(), twb.end());
+  for (int q = 0; q <= L / 2; q++) ps.tw.push_back(expi_pi(q, 2LL * L));  // e^{i pi q / 2L}
+  a.ntw_a = (int)twa.size();
+  a.ntw_b = (int)twb.size();
+  a.ntw = (int)ps.tw.size();
+  ps.wN.clear();
+  return true;
+}
+
 // Switch an already described pass to the compile-time-length path when it
 // is a C2C, R2C, C2R or complex DCT-I whose complex length M is one of the
 // fast lengths (fast_radices: powers of two 16..4096, 3 and 5 times a power
@@ -295,9 +362,12 @@ static void setup_fast(int prec, const OpSpec &op, PassDesc &ps, int batch_threa
   int M;
   if (op.cls == CLS_C2C) M = a.N;
   else if ((op.cls == CLS_R2C || op.cls == CLS_C2R) && a.N % 2 == 0) M = a.N / 2;
-  // DCT-I of complex lines: the M = 2 (N - 1)-point FFT of the even extension
-  else if (op.cls == CLS_R2R && op.r2r_kind == REDFT00 && a.in_cplx && a.out_cplx) M = a.W;
-  else return;
+  // DCT-I of complex lines: the split form, or the M = 2 (N - 1)-point FFT
+  // of the even extension
+  else if (op.cls == CLS_R2R && op.r2r_kind == REDFT00 && a.in_cplx && a.out_cplx) {
+    if (batch_threads == 0 && g_dct_split && setup_dct1_split(prec, ps)) return;
+    M = a.W;
+  } else return;
   const bool split = op.cls == CLS_R2C || op.cls == CLS_C2R;
   int rad[P3E_MAX_RADIX_PASSES];
   const int nrad = fast_radices(M, rad);
@@ -403,6 +473,7 @@ static void setup_fast(int prec, const OpSpec &op, PassDesc &ps, int batch_threa
   a.ntile_e = (a.ne + B - 1) / B;
   a.div_ntile = make_fastdiv((uint32_t)a.ntile_e);
   a.div_pairs = make_fastdiv((uint32_t)(M / 2 + 1));
+  a.tab_off = B * lp;
   ps.lds_bytes = lds(B);
   ps.threads = M * B / P3E_FAST_E;
   a.nthreads = ps.threads;
@@ -744,7 +815,8 @@ bool plan_chain(int prec, const OpSpec ops[3], const int order[3], const Layout 
     else
       snprintf(kind, sizeof(kind), "%s",
                ps.args.bluestein ? (ps.args.mid ? ", bluestein mid" : ", bluestein")
-                                 : (ps.args.fast ? ", pow2" : (ps.args.mid ? ", mid" : "")));
+                                 : (ps.args.fast ? (ps.args.dct_split ? ", pow2 split" : ", pow2")
+                                                 : (ps.args.mid ? ", mid" : "")));
     if (ps.args.pair) strncat(kind, " pair", sizeof(kind) - strlen(kind) - 1);
     char pitch[24] = "";
     if (lay[i + 1].pitch > 0) snprintf(pitch, sizeof(pitch), ", out pitch %d", lay[i + 1].pitch);

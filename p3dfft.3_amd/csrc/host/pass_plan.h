@@ -50,6 +50,9 @@ struct PassDesc {
 
 // Disable the power-of-two path (tests compare both paths).
 void set_fast_path(bool enable);
+// Test hook: plan complex DCT-I passes of 129, 257, 513 and 1025 points in
+// the split form (default) or as the FFT of the even extension.
+void set_dct_split(bool on);
 // Intermediate arrays of at least this many bytes whose stride-1 rows are a
 // multiple of 256 bytes get padded rows (tests lower it to reach small
 // grids).

@@ -142,6 +142,46 @@ P3E_HD P3E_INL void fft_fixed(Run &run, cx<T> *buf, int lp, const cx<T> *tw, int
   else fft_seq<T, M, 1, S, Run, 16, 8, 5>(run, buf, lp, tw, B);  // 640
 }
 
+// Two radix sequences run side by side, pass for pass (the split DCT-I):
+// sequence A (length MA, sign SA) on threads 0 .. nthA - 1 over the lines
+// at bufA, sequence B (length MB, sign SB) on the next nthB threads over the
+// lines at bufB; `run.pair` executes one pass of each between the same
+// barriers.  Both sequences have the same number of passes.
+template <int... R> struct Rads {};
+
+template <typename T, int MA, int MB, int NsA, int NsB, int SA, int SB, typename Run, int RA, int... RAs, int RB,
+          int... RBs>
+P3E_HD P3E_INL void dual_seq(Run &run, Rads<RA, RAs...>, Rads<RB, RBs...>, cx<T> *bufA, int lpA, const cx<T> *twA,
+                             int nthA, cx<T> *bufB, int lpB, const cx<T> *twB, int nthB, int B) {
+  static_assert(sizeof...(RAs) == sizeof...(RBs), "sequences of different lengths");
+  FPass<T, RA, MA, NsA, SA> pa{bufA, lpA, twA, B * (MA / RA)};
+  FPass<T, RB, MB, NsB, SB> pb{bufB, lpB, twB, B * (MB / RB)};
+  run.pair(pa, nthA, pb, nthB);
+  if constexpr (sizeof...(RAs) > 0)
+    dual_seq<T, MA, MB, NsA * RA, NsB * RB, SA, SB, Run>(run, Rads<RAs...>{}, Rads<RBs...>{}, bufA, lpA,
+                                                         twA + NsA * tw_stride(RA), nthA, bufB, lpB,
+                                                         twB + NsB * tw_stride(RB), nthB, B);
+}
+
+// The radices of dct1_radices (pass_args.h) as compile-time sequences.
+template <int P> struct Dct1Rads;
+template <> struct Dct1Rads<128> {
+  using A = Rads<16, 8>;
+  using B = Rads<8, 8>;
+};
+template <> struct Dct1Rads<256> {
+  using A = Rads<16, 16>;
+  using B = Rads<16, 8>;
+};
+template <> struct Dct1Rads<512> {
+  using A = Rads<8, 8, 8>;
+  using B = Rads<8, 8, 4>;
+};
+template <> struct Dct1Rads<1024> {
+  using A = Rads<16, 8, 8>;
+  using B = Rads<8, 8, 8>;
+};
+
 // Tile coordinates of workgroup `tile`.
 P3E_HD P3E_INL void fast_tile_coords(const PassArgs &p, uint32_t tile, uint32_t *f, int *e0, int *nb) {
   *f = fdiv(tile, p.div_ntile);
@@ -673,6 +713,122 @@ P3E_HD P3E_INL void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, co
   }
 }
 
+// ------------------------------------------------------------ split DCT-I
+// Complex DCT-I (REDFT00) of P + 1 points, P = 2L, as two transforms of half
+// the size of the 2P-point FFT of the even extension:
+//   even outputs  Y_2m = DFT_P(ext a)_m, m = 0..L, where a_n = x_n + x_{P-n}
+//     (n = 0..L) and ext a is its even extension (a DCT-I of L + 1 points);
+//   odd outputs   Y_2m+1 = REDFT01_L(c)_m, m = 0..L-1, c_n = x_n - x_{P-n},
+//     computed as v = IDFT_L(V) with V_n = (c_n - i c_{L-n}) e^{i pi n/2L}
+//     (c_L = 0): Y_4j+1 = v_j, Y_4j+3 = v_{L-1-j} (Makhoul's DCT-III by an
+//     L-point FFT; every step is complex-linear, so it holds for complex x).
+// The load phase reads the quad x_q, x_{L-q}, x_{L+q}, x_{P-q} (q = 0..L/2)
+// in one thread and writes a and V at both ends: V_{L-q} = (c_q + i c_{L-q})
+// conj(e^{i pi q/2L}), so the pre-twiddle table has L/2 + 1 entries.
+// Region A (B lines of pitch lp) holds ext a, region B (B lines of pitch
+// lp2) holds V; tab = [twiddles of A (ntw_a) | of B (ntw_b) | pre-twiddles].
+template <typename T, int P, bool FULL>
+P3E_HD P3E_INL void dct1_load(const PassArgs &p, uint32_t f, int e0, int nb, cx<T> *bufA, cx<T> *bufB,
+                              const cx<T> *pre, int tid, int nth) {
+  constexpr int L = P / 2, Q = L / 2 + 1;
+  // quads per thread: B Q quads over (P + L) B / E threads
+  constexpr int QN = (Q * P3E_FAST_E + P + L - 1) / (P + L);
+  constexpr uint32_t CS = 2 * sizeof(T);
+  const char *tin = reinterpret_cast<const char *>(p.in) + ((long long)e0 * p.is_e + (long long)f * p.is_f) * CS;
+  const int isd = (int)p.is_d, ise = (int)p.is_e;
+  const int tot = (p.load_order == ORD_LINE ? nb : p.B) * Q;
+  cx<T> x[QN][4];
+  int jq[QN][2];
+#pragma unroll
+  for (int u = 0; u < QN; u++) {
+    const int i = tid + u * nth;
+    int j, q;
+    if (p.load_order == ORD_LINE) {
+      j = qdiv<Q>(i);
+      q = qmod<Q>(i);
+    } else {
+      j = i & (p.B - 1);
+      q = i >> p.logB;
+    }
+    const bool ok = i < tot && (FULL || j < nb);
+    jq[u][0] = ok ? j : -1;
+    jq[u][1] = q;
+    const uint32_t o = ok ? (uint32_t)(j * ise) : 0u;
+    const uint32_t qq = ok ? (uint32_t)q : 0u;
+    x[u][0] = ldg<cx<T>>(tin, (o + qq * isd) * CS);
+    x[u][1] = ldg<cx<T>>(tin, (o + (L - qq) * isd) * CS);
+    x[u][2] = ldg<cx<T>>(tin, (o + (L + qq) * isd) * CS);
+    x[u][3] = ldg<cx<T>>(tin, (o + (P - qq) * isd) * CS);
+  }
+#pragma unroll
+  for (int u = 0; u < QN; u++) {
+    const int j = jq[u][0], q = jq[u][1];
+    if (j < 0) continue;
+    const cx<T> aq = add(x[u][0], x[u][3]), al = add(x[u][1], x[u][2]);
+    const cx<T> cq = sub(x[u][0], x[u][3]), cl = sub(x[u][1], x[u][2]);
+    cx<T> *A = bufA + j * p.lp;
+    A[P3E_PAD(q)] = aq;
+    A[P3E_PAD(L - q)] = al;
+    A[P3E_PAD(L + q)] = al;
+    if (q > 0) A[P3E_PAD(P - q)] = aq;
+    const cx<T> t = pre[q];
+    cx<T> *V = bufB + j * p.lp2;
+    V[P3E_PAD(q)] = mul(add(cq, rot(cl, -1)), t);
+    if (q > 0) V[P3E_PAD(L - q)] = mul(add(cq, rot(cl, 1)), conj(t));
+  }
+}
+
+template <typename T, int P, bool FULL>
+P3E_HD P3E_INL void dct1_store(const PassArgs &p, uint32_t f, int e0, int nb, const cx<T> *bufA, const cx<T> *bufB,
+                               int tid, int nth) {
+  constexpr int L = P / 2;
+  constexpr uint32_t CS = 2 * sizeof(T);
+  char *tout = reinterpret_cast<char *>(p.out) + ((long long)e0 * p.os_e + (long long)f * p.os_f) * CS;
+  const int osd = (int)p.os_d, ose = (int)p.os_e;
+  const int tot = (p.store_order == ORD_LINE ? nb : p.B) * (P + 1);
+#pragma unroll 4
+  for (int i = tid; i < tot; i += nth) {
+    int j, k;
+    if (p.store_order == ORD_LINE) {
+      j = qdiv<P + 1>(i);
+      k = qmod<P + 1>(i);
+    } else {
+      j = i & (p.B - 1);
+      k = i >> p.logB;
+      if (!FULL && j >= nb) continue;
+    }
+    // Y_2m from A; Y_4j+1 = v_j, Y_4j+3 = v_{L-1-j} from B
+    const int vi = (k & 2) ? L - 1 - (k >> 2) : (k >> 2);
+    const cx<T> y = (k & 1) ? bufB[j * p.lp2 + P3E_PAD(vi)] : bufA[j * p.lp + P3E_PAD(k >> 1)];
+    stg(tout, (uint32_t)(k * osd + j * ose) * CS, out_rule(p, y, k, e0 + j, f));
+  }
+}
+
+template <typename T, int P, typename Run, typename Each>
+P3E_HD P3E_INL void dct1_tile(const PassArgs &p, uint32_t tile, cx<T> *buf, int nth, Run &run, Each &each,
+                              const cx<T> *tab) {
+  constexpr int L = P / 2;
+  uint32_t f;
+  int e0, nb;
+  fast_tile_coords(p, tile, &f, &e0, &nb);
+  const bool full = nb == p.B;
+  cx<T> *bufA = buf, *bufB = buf + p.B * p.lp;
+  const cx<T> *twA = tab, *twB = tab + p.ntw_a, *pre = tab + p.ntw_a + p.ntw_b;
+  each([&](int tid) P3E_LAMBDA_INL {
+    if (tid >= nth) return;
+    if (full) dct1_load<T, P, true>(p, f, e0, nb, bufA, bufB, pre, tid, nth);
+    else dct1_load<T, P, false>(p, f, e0, nb, bufA, bufB, pre, tid, nth);
+  });
+  const int nthA = P * p.B / P3E_FAST_E, nthB = L * p.B / P3E_FAST_E;
+  dual_seq<T, P, L, 1, 1, -1, 1, Run>(run, typename Dct1Rads<P>::A{}, typename Dct1Rads<P>::B{}, bufA, p.lp, twA,
+                                      nthA, bufB, p.lp2, twB, nthB, p.B);
+  each([&](int tid) P3E_LAMBDA_INL {
+    if (tid >= nth) return;
+    if (full) dct1_store<T, P, true>(p, f, e0, nb, bufA, bufB, tid, nth);
+    else dct1_store<T, P, false>(p, f, e0, nb, bufA, bufB, tid, nth);
+  });
+}
+
 // Whole tile.  `each(f)` runs phase f for the calling thread and ends with a
 // barrier (device), or runs it for every thread of the workgroup in turn
 // (host build); `run` does the same for the two halves of a radix pass.
@@ -684,6 +840,12 @@ P3E_HD P3E_INL void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, co
 template <typename T, int M, typename Run, typename Each>
 P3E_HD P3E_INL void fast_tile(const PassArgs &p, uint32_t tile, cx<T> *buf, int nth, Run &run, Each &each,
                               const cx<T> *tw, const cx<T> *wN) {
+  if constexpr (M == 128 || M == 256 || M == 512 || M == 1024) {
+    if (p.dct_split) {
+      dct1_tile<T, M>(p, tile, buf, nth, run, each, tw);
+      return;
+    }
+  }
   uint32_t f;
   int e0, nb;
   fast_tile_coords(p, tile, &f, &e0, &nb);

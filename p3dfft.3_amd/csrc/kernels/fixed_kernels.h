@@ -22,7 +22,7 @@ template <typename T, int M>
 __device__ __forceinline__ void pow2_body(const PassArgs &p, uint32_t tile) {
   extern __shared__ unsigned char smem_raw[];
   cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
-  cx<T> *ltw = buf + (size_t)p.B * p.lp;
+  cx<T> *ltw = buf + p.tab_off;
   cx<T> *lwn = ltw + p.ntw;
   const cx<T> *gtw = reinterpret_cast<const cx<T> *>(p.tw);
   const cx<T> *gwn = reinterpret_cast<const cx<T> *>(p.wN);

@@ -97,6 +97,15 @@ struct PassArgs {
   // order (pass_dev.h, xcd_tile), for tiles whose tile-order rows are
   // shorter than a 128-byte cache line
   int xcd_map;
+  // offset of the constant tables in the kernel's LDS, in complex elements
+  // (after the tile)
+  int tab_off;
+  // complex DCT-I of M + 1 points split into an M-point FFT and an
+  // M/2-point inverse FFT (fft_fast.h, dct1_tile): the second one's lines
+  // have LDS pitch lp2 after the first one's; the tables are the two
+  // twiddle tables (ntw_a and ntw_b entries) and M/4 + 1 pre-twiddles
+  int dct_split;
+  int lp2, ntw_a, ntw_b;
 };
 
 // One entry of a batch work list: tile `tile` of the pass table[pass].
@@ -163,6 +172,18 @@ static inline int fast_radices(int M, int *rad) {
     case 1024: rad[0] = 16; rad[1] = 8; rad[2] = 8; return 3;
     case 2048: rad[0] = 16; rad[1] = 16; rad[2] = 8; return 3;
     case 4096: rad[0] = 16; rad[1] = 16; rad[2] = 16; return 3;
+  }
+  return 0;
+}
+// Radix sequences of the split complex DCT-I of P + 1 points (fft_fast.h,
+// dct1_tile): the P-point FFT (ra) and the P/2-point inverse FFT (rb), the
+// same number of passes; 0 if P has no split form.
+static inline int dct1_radices(int P, int *ra, int *rb) {
+  switch (P) {
+    case 128: ra[0] = 16; ra[1] = 8; rb[0] = 8; rb[1] = 8; return 2;
+    case 256: ra[0] = 16; ra[1] = 16; rb[0] = 16; rb[1] = 8; return 2;
+    case 512: ra[0] = 8; ra[1] = 8; ra[2] = 8; rb[0] = 8; rb[1] = 8; rb[2] = 4; return 3;
+    case 1024: ra[0] = 16; ra[1] = 8; ra[2] = 8; rb[0] = 8; rb[1] = 8; rb[2] = 8; return 3;
   }
   return 0;
 }

@@ -33,6 +33,19 @@ template <typename T> struct DevPassRun {
     if (tid < nth) ps.write(tid, nth, v);
     __syncthreads();
   }
+  // one pass of each of two sequences: threads 0 .. na - 1 run pa, the next
+  // nb run pb (fft_fast.h, dual_seq)
+  template <class PA, class PB> __device__ __forceinline__ void pair(PA &pa, int na, PB &pb, int nb) const {
+    constexpr int S = PA::SLOTS > PB::SLOTS ? PA::SLOTS : PB::SLOTS;
+    cx<T> v[S];
+    const bool in_a = tid < na, in_b = !in_a && tid - na < nb;
+    if (in_a) pa.read(tid, na, v);
+    else if (in_b) pb.read(tid - na, nb, v);
+    __syncthreads();
+    if (in_a) pa.write(tid, na, v);
+    else if (in_b) pb.write(tid - na, nb, v);
+    __syncthreads();
+  }
 };
 
 // A plain phase of a tile for this thread, then a barrier.

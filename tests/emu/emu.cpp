@@ -31,6 +31,9 @@ template <typename T> static std::vector<T> tab(const std::vector<cld> &v) {
 
 // Every thread of the workgroup is emulated for a radix pass: all read
 // halves, then all write halves, each thread keeping its own registers.
+// set when a split pass asks for more threads than its workgroup has
+static bool g_pair_overflow = false;
+
 template <typename T> struct HostPassRun {
   int nth;
   std::vector<cx<T>> v;
@@ -38,6 +41,18 @@ template <typename T> struct HostPassRun {
     v.assign((size_t)nth * PS::SLOTS, cx<T>{(T)1e30, (T)-1e30});
     for (int tid = 0; tid < nth; tid++) ps.read(tid, nth, &v[(size_t)tid * PS::SLOTS]);
     for (int tid = 0; tid < nth; tid++) ps.write(tid, nth, &v[(size_t)tid * PS::SLOTS]);
+  }
+  template <class PA, class PB> void pair(PA &pa, int na, PB &pb, int nb) {
+    if (na + nb > nth) {
+      g_pair_overflow = true;
+      return;
+    }
+    std::vector<cx<T>> va((size_t)na * PA::SLOTS, cx<T>{(T)1e30, (T)-1e30});
+    std::vector<cx<T>> vb((size_t)nb * PB::SLOTS, cx<T>{(T)1e30, (T)-1e30});
+    for (int tid = 0; tid < na; tid++) pa.read(tid, na, &va[(size_t)tid * PA::SLOTS]);
+    for (int tid = 0; tid < nb; tid++) pb.read(tid, nb, &vb[(size_t)tid * PB::SLOTS]);
+    for (int tid = 0; tid < na; tid++) pa.write(tid, na, &va[(size_t)tid * PA::SLOTS]);
+    for (int tid = 0; tid < nb; tid++) pb.write(tid, nb, &vb[(size_t)tid * PB::SLOTS]);
   }
 };
 
@@ -60,7 +75,7 @@ template <typename T, int M> static int run_pow2(const PassDesc &pd, const PassA
   HostPassRun<T> run{pd.threads, std::vector<cx<T>>((size_t)pd.threads * P3E_FAST_E)};
   HostEach each{pd.threads};
   const size_t n_lds = pd.lds_bytes / sizeof(cx<T>), guard = 64;
-  const size_t tile = (size_t)a.B * a.lp;
+  const size_t tile = (size_t)a.tab_off;
   const size_t ntab = g_batch_plans ? 0 : (size_t)a.ntw + (a.cls == CLS_R2C || a.cls == CLS_C2R ? M + 1 : 0);
   if (tile + ntab > n_lds) {
     set_error("power-of-two pass: tile and tables need %zu entries, LDS holds %zu",
@@ -80,6 +95,11 @@ template <typename T, int M> static int run_pow2(const PassDesc &pd, const PassA
   for (unsigned int t = 0; t < pd.nblocks; t++) {
     for (size_t i = 0; i < tile; i++) lds[i] = poison;
     fast_tile<T, M>(a, t, lds.data(), pd.threads, run, each, g_batch_plans ? gtw : ltw, g_batch_plans ? gwn : lwn);
+  }
+  if (g_pair_overflow) {
+    g_pair_overflow = false;
+    set_error("split pass: its two sequences need more threads than the workgroup has");
+    return -1;
   }
   for (size_t i = n_lds; i < n_lds + guard; i++)
     if (lds[i].x != poison.x || lds[i].y != poison.y) {
@@ -286,6 +306,7 @@ const char *emu_error(void) { return last_error(); }
 // 1: plan with the power-of-two path where eligible (the product default);
 // 0: generic path only.
 void emu_set_fast(int enable) { set_fast_path(enable != 0); }
+void emu_set_dct_split(int on) { set_dct_split(on != 0); }
 void emu_set_pad_threshold(long long bytes) { set_pad_threshold((size_t)bytes); }
 
 // Buffer assignment of a plan, for tests: dst[i] per pass, scratch sizes.

@@ -28,6 +28,12 @@ def lib():
     return _lib
 
 
+def set_dct_split(on):
+    """Plan complex DCT-I passes of 129/257/513/1025 points in the split form
+    (the default) or as the FFT of the even extension."""
+    lib().emu_set_dct_split(1 if on else 0)
+
+
 def set_pad_threshold(nbytes):
     """Intermediate arrays of at least nbytes get padded rows (the planner's
     default is 32 MiB; 0 pads every eligible intermediate)."""

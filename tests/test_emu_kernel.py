@@ -406,3 +406,35 @@ def test_fixed_length_dct1_complex(n, prec):
             assert f"r2r[axis {ax}, N={n}, pow2" in desc, desc
         ref = orc.exec_1d(C.r2r_id("DCT1", True, 8), ax, dims, mo1, dims, mo2, x)
         assert C.rel_l2(got, ref) < TOL[prec], desc
+
+
+@pytest.mark.parametrize("n", [129, 257, 513, 1025])
+@pytest.mark.parametrize("prec", [4, 8])
+def test_dct1_complex_split(n, prec):
+    """Complex DCT-I of n = P + 1 points in the split form (a P-point FFT of
+    the even extension of x_n + x_{P-n} and a P/2-point inverse FFT for the
+    odd outputs): line order, tile order with whole and partial tiles,
+    against the oracle and against the unsplit form."""
+    tid = C.r2r_id("DCT1", True, prec)
+    empty = C.EMPTY_DC if prec == 8 else C.EMPTY_SC
+    cases = [([n, 5, 3], [0, 1, 2], [0, 1, 2]),   # line order both sides
+             ([16, 3, n], [0, 1, 2], [0, 1, 2]),  # tile order, whole tiles
+             ([7, 3, n], [0, 1, 2], [2, 0, 1]),   # tile load (partial tile), line store
+             ([n, 5, 4], [1, 2, 0], [0, 2, 1])]   # tile load, line store
+    rng = np.random.default_rng(n + prec)
+    for dims, mo1, mo2 in cases:
+        shp = orc.storage_shape(dims, mo1)
+        x = rng.standard_normal(shp) + 1j * rng.standard_normal(shp)
+        ax = dims.index(n)
+        types = [empty if a != ax else tid for a in range(3)]
+        got, desc = emu.exec3d(types, dims, mo1, dims, mo2, x, prec)
+        assert f"r2r[axis {ax}, N={n}, pow2 split" in desc, desc
+        ref = orc.exec_1d(C.r2r_id("DCT1", True, 8), ax, dims, mo1, dims, mo2, x)
+        assert C.rel_l2(got, ref) < TOL[prec], desc
+        emu.set_dct_split(False)
+        try:
+            plain, desc_p = emu.exec3d(types, dims, mo1, dims, mo2, x, prec)
+        finally:
+            emu.set_dct_split(True)
+        assert "split" not in desc_p, desc_p
+        assert C.rel_l2(got, plain) < TOL[prec], (desc, desc_p)
