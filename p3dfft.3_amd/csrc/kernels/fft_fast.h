@@ -655,6 +655,23 @@ P3E_HD P3E_INL void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, co
 #pragma unroll
     for (int q = 0; q < 2 * E; q++)
       if (off[q] >= 0) stg(tout, (uint32_t)off[q] * RS, p.has_scale ? y[q] * (T)p.scale : y[q]);
+  } else if (p.cls == CLS_R2C && p.store_order == ORD_LINE) {
+    // R2C, line order (d stride-1): outputs k and M - k of line j from one
+    // pair (Z_k, Z_{M-k}) of the packed spectrum: with e = Z_k + conj Z_{M-k}
+    // and t = w_N^k (Z_k - conj Z_{M-k}), X_k = (e - i t)/2 and
+    // X_{M-k} = conj(e + i t)/2 (as w_N^{M-k} = -conj w_N^k)
+    const int ptot = nb * (M / 2 + 1);
+#pragma unroll 2
+    for (int i = tid; i < ptot; i += nth) {
+      const int j = qdiv<M / 2 + 1>(i), k = qmod<M / 2 + 1>(i);
+      const cx<T> *l = buf + j * lp;
+      const cx<T> zk = l[P3E_PAD(k)], zm = l[P3E_PAD(k == 0 ? 0 : M - k)];
+      const cx<T> e = add(zk, conj(zm)), t = mul(wN[k], sub(zk, conj(zm)));
+      const uint32_t lo = (uint32_t)(j * ose);
+      stg(tout, (lo + (uint32_t)k) * CS, out_rule(p, scl(add(e, rot(t, -1)), (T)0.5), k, e0 + j, f));
+      if (2 * k != M)
+        stg(tout, (lo + (uint32_t)(M - k)) * CS, out_rule(p, scl(conj(add(e, rot(t, 1))), (T)0.5), M - k, e0 + j, f));
+    }
   } else if (p.vec2s && p.store_order == ORD_LINE) {
     // C2C, line order (d stride-1): elements k = 2 kp and k + 1 of line j
 #pragma unroll
