@@ -583,6 +583,48 @@ P3E_HD P3E_INL void store_pair_tile(const PassArgs &p, uint32_t f, int e0, const
   stg(tout, (uint32_t)(k * (int)p.os_d + j) * (uint32_t)(2 * sizeof(T)), v);
 }
 
+// R2C outputs k and M - k of one line from the packed pair (Z_k, Z_{M-k}):
+// with e = Z_k + conj Z_{M-k} and t = w_N^k (Z_k - conj Z_{M-k}),
+// X_k = (e - i t)/2 and X_{M-k} = conj(e + i t)/2 (w_N^{M-k} = -conj w_N^k).
+template <typename T, int M>
+P3E_HD P3E_INL void r2c_split2(const cx<T> *l, const cx<T> *wN, int k, cx<T> *xk, cx<T> *xm) {
+  const cx<T> zk = l[P3E_PAD(k)], zm = l[P3E_PAD(k == 0 ? 0 : M - k)];
+  const cx<T> e = add(zk, conj(zm)), t = mul(wN[k], sub(zk, conj(zm)));
+  *xk = scl(add(e, rot(t, -1)), (T)0.5);
+  *xm = scl(conj(add(e, rot(t, 1))), (T)0.5);
+}
+
+// Tile-order paired R2C store of entry i (e stride-1, whole tiles): outputs
+// k and M - k (k = 0..M/2) of lines j = 2 jp and j + 1, adjacent in memory,
+// by one store each.
+template <typename T, int M>
+P3E_HD P3E_INL void store_quad_tile(const PassArgs &p, uint32_t f, int e0, const cx<T> *buf, const cx<T> *wN,
+                                    char *tout, int i) {
+  const int lb = p.logB - 1, lp = p.lp;
+  const int j = 2 * (i & ((1 << lb) - 1)), k = i >> lb;
+  cx<T> a0, a1, b0, b1;
+  r2c_split2<T, M>(buf + j * lp, wN, k, &a0, &b0);
+  r2c_split2<T, M>(buf + (j + 1) * lp, wN, k, &a1, &b1);
+  a0 = out_rule(p, a0, k, e0 + j, f);
+  a1 = out_rule(p, a1, k, e0 + j + 1, f);
+  CxPair<T> v;
+  v.ar = a0.x;
+  v.ai = a0.y;
+  v.br = a1.x;
+  v.bi = a1.y;
+  const int osd = (int)p.os_d;
+  stg(tout, (uint32_t)(k * osd + j) * (uint32_t)(2 * sizeof(T)), v);
+  if (2 * k != M) {
+    b0 = out_rule(p, b0, M - k, e0 + j, f);
+    b1 = out_rule(p, b1, M - k, e0 + j + 1, f);
+    v.ar = b0.x;
+    v.ai = b0.y;
+    v.br = b1.x;
+    v.bi = b1.y;
+    stg(tout, (uint32_t)((M - k) * osd + j) * (uint32_t)(2 * sizeof(T)), v);
+  }
+}
+
 // Store phase: output rule and LDS -> tile.
 template <typename T, int M, bool FULL>
 P3E_HD P3E_INL void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, const cx<T> *buf, int tid, int nth,
@@ -664,13 +706,11 @@ P3E_HD P3E_INL void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, co
 #pragma unroll 2
     for (int i = tid; i < ptot; i += nth) {
       const int j = qdiv<M / 2 + 1>(i), k = qmod<M / 2 + 1>(i);
-      const cx<T> *l = buf + j * lp;
-      const cx<T> zk = l[P3E_PAD(k)], zm = l[P3E_PAD(k == 0 ? 0 : M - k)];
-      const cx<T> e = add(zk, conj(zm)), t = mul(wN[k], sub(zk, conj(zm)));
+      cx<T> xk, xm;
+      r2c_split2<T, M>(buf + j * lp, wN, k, &xk, &xm);
       const uint32_t lo = (uint32_t)(j * ose);
-      stg(tout, (lo + (uint32_t)k) * CS, out_rule(p, scl(add(e, rot(t, -1)), (T)0.5), k, e0 + j, f));
-      if (2 * k != M)
-        stg(tout, (lo + (uint32_t)(M - k)) * CS, out_rule(p, scl(conj(add(e, rot(t, 1))), (T)0.5), M - k, e0 + j, f));
+      stg(tout, (lo + (uint32_t)k) * CS, out_rule(p, xk, k, e0 + j, f));
+      if (2 * k != M) stg(tout, (lo + (uint32_t)(M - k)) * CS, out_rule(p, xm, M - k, e0 + j, f));
     }
   } else if (p.vec2s && p.store_order == ORD_LINE) {
     // C2C, line order (d stride-1): elements k = 2 kp and k + 1 of line j
@@ -693,7 +733,11 @@ P3E_HD P3E_INL void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, co
     // j = 2 jp and j + 1, one store.  (E/2) nth entries are exactly the C2C
     // tile (nout = M); an R2C tile has nout = M + 1, B/2 entries more.
     const int ptot = (p.B >> 1) * p.nout;
-    if (p.cls == CLS_R2R) {
+    if (p.cls == CLS_R2C) {
+      // outputs (k, M - k) of two lines per entry: (B/2)(M/2 + 1) entries
+      const int qtot = (p.B >> 1) * (M / 2 + 1);
+      for (int i = tid; i < qtot; i += nth) store_quad_tile<T, M>(p, f, e0, buf, wN, tout, i);
+    } else if (p.cls == CLS_R2R) {
       // DCT-I: nout = M/2 + 1, B nout / 2 pairs in E/4 + 1 entries
 #pragma unroll
       for (int q = 0; q < E / 4 + 1; q++) {

@@ -438,3 +438,20 @@ def test_dct1_complex_split(n, prec):
             emu.set_dct_split(True)
         assert "split" not in desc_p, desc_p
         assert C.rel_l2(got, plain) < TOL[prec], (desc, desc_p)
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+@pytest.mark.parametrize("dims", [[64, 4, 16], [512, 2, 64], [96, 3, 32], [768, 2, 8]])
+def test_r2c_tile_order_paired_store(dims, prec):
+    """R2C along the slowest axis with a stride-1 tile axis and whole tiles
+    (the paired tile-order store: outputs k and M - k of two adjacent lines
+    per entry), against the oracle."""
+    mo = [2, 1, 0]
+    x = np.random.default_rng(dims[0]).uniform(-1, 1, orc.storage_shape(dims, mo))
+    r2c = C.R2C_D if prec == 8 else C.R2C_S
+    cf = C.CF_D if prec == 8 else C.CF_S
+    od = [dims[0] // 2 + 1, dims[1], dims[2]]
+    got, desc = emu.exec3d([r2c, cf, cf], dims, mo, od, mo, x, prec)
+    assert f"r2c[axis 0, N={dims[0]}, pow2" in desc, desc
+    ref = orc.exec3d([C.R2C_D, C.CF_D, C.CF_D], dims, mo, mo, x)
+    assert C.rel_l2(got, ref) < TOL[prec], desc
