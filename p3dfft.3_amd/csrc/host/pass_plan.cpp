@@ -200,6 +200,12 @@ static const size_t LDS_BUDGET = 80 * 1024;
 static const size_t LDS_MAX = 160 * 1024;
 
 
+// LDS line pitch of the compile-time-length path (setup_fast)
+static int fast_pitch(int prec, int M) {
+  const int g = prec == 8 ? 16 : 32;
+  return ((M + M / 16 + g - 1) / g) * g + 1;
+}
+
 static bool g_fast_enabled = true;
 static bool g_dct_split = true;
 void set_dct_split(bool on) { g_dct_split = on; }
@@ -298,7 +304,7 @@ static bool setup_dct1_split(int prec, PassDesc &ps) {
   const int nrad = dct1_radices(P, ra, rb);
   if (nrad == 0) return false;
   const size_t esz = 2 * (size_t)prec;
-  const int lp = ((P + P / 16 + 31) & ~31) + 1, lp2 = ((L + L / 16 + 31) & ~31) + 1;
+  const int lp = fast_pitch(prec, P), lp2 = fast_pitch(prec, L);
   std::vector<cld> twa, twb;
   pass_twiddles(ra, nrad, twa);
   pass_twiddles(rb, nrad, twb);
@@ -376,11 +382,14 @@ static void setup_fast(int prec, const OpSpec &op, PassDesc &ps, int batch_threa
   if (batch_threads > 0 && !is_pow2(M)) return;
   const size_t esz = 2 * (size_t)prec;
   // Line pitch in LDS: room for the padded line (P3E_PAD(M-1) < M + M/16),
-  // rounded up to 1 mod 32 complex elements.  The load and store phases of
-  // tile order touch one element of each of many lines at once; a pitch of
-  // 1 mod 32 puts consecutive lines on consecutive LDS bank pairs (FP32) or
-  // quads (FP64) instead of on the same banks.
-  const int lp = ((M + M / 16 + 31) & ~31) + 1;
+  // rounded up to 1 mod 32 complex elements (FP32) or 1 mod 16 (FP64).  The
+  // load and store phases of tile order touch one element of each of many
+  // lines at once; such a pitch puts consecutive lines on consecutive LDS
+  // bank pairs (FP32: 32 lines round the 64 banks) or quads (FP64: 16 lines)
+  // instead of on the same banks.  The FP64 rounding is the smaller one: it
+  // lets a 256-point FP64 tile of 16 lines and its tables fit 80 KiB
+  // (128-byte rows of reals for the R2C/C2R passes of a 512^3 grid).
+  const int lp = fast_pitch(prec, M);
   const long maxth = batch_threads > 0 ? batch_threads : P3E_FAST_MAX_THREADS;
   int B = 1;
   // Tile width: when the tile axis e is strided on one side (tile-order load
