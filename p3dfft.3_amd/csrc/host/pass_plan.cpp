@@ -719,6 +719,28 @@ static int pass_cost(int axis, const int moi[3], const int moo[3]) {
 
 static bool same_mo(const int a[3], const int b[3]) { return a[0] == b[0] && a[1] == b[1] && a[2] == b[2]; }
 
+// Secondary cost of a pass along `axis` from layout lin to lout (memory
+// orders as given): 1 when a side is tile order (the axis strided there) and
+// the tile axis e (chosen as describe_pass chooses it) has an extent that
+// is not a multiple of 8.  Such a pass has a partial tile in every row and
+// no paired 16-byte accesses; on the 768x640x513 R2C / DCT-I / C2C chain
+// (tile axes of 385 complex values) the candidate orders with one such pass
+// instead of two ran 0.15-0.17 ms faster in each direction than the
+// others of equal primary cost (1.45-1.53 ms against 1.62-1.88 ms for the
+// three passes).
+static int ragged_tile_cost(int axis, const Layout &lin, const Layout &lout) {
+  long long is[3], os[3];
+  strides_of(lin, is);
+  strides_of(lout, os);
+  const bool tile_in = is[axis] != 1, tile_out = os[axis] != 1;
+  if (!tile_in && !tile_out) return 0;
+  const int o1 = (axis + 1) % 3, o2 = (axis + 2) % 3;
+  int e;
+  if (tile_in) e = (is[o1] <= is[o2]) ? o1 : o2;
+  else e = (os[o1] <= os[o2]) ? o1 : o2;
+  return lin.dims[e] % 8 != 0 ? 1 : 0;
+}
+
 bool plan_chain(int prec, const OpSpec ops[3], const int order[3], const Layout &in, const Layout &out,
                 PassChain &chain, int batch_threads) {
   chain.passes.clear();
@@ -784,7 +806,7 @@ bool plan_chain(int prec, const OpSpec ops[3], const int order[3], const Layout 
     else cand.push_back(q);
   int ncomb = 1;
   for (int i = 1; i < k; i++) ncomb *= 6;
-  int best_cost = 1 << 30;
+  int best_cost = 1 << 30, best_ragged = 1 << 30;
   std::vector<int> best(k + 1, 0);
   for (int c = 0; c < ncomb; c++) {
     int cc = c;
@@ -799,8 +821,19 @@ bool plan_chain(int prec, const OpSpec ops[3], const int order[3], const Layout 
     }
     int cost = 0;
     for (int i = 0; i < k; i++) cost += pass_cost(axes[i], mo[i], mo[i + 1]);
-    if (cost < best_cost) {
+    if (cost > best_cost) continue;
+    // ties: fewest passes with a ragged tile axis
+    int ragged = 0;
+    for (int i = 0; i < k; i++) {
+      Layout a = lay[i], b = lay[i + 1];
+      memcpy(a.mo, mo[i], sizeof(a.mo));
+      memcpy(b.mo, mo[i + 1], sizeof(b.mo));
+      a.pitch = b.pitch = 0;
+      ragged += ragged_tile_cost(axes[i], a, b);
+    }
+    if (cost < best_cost || ragged < best_ragged) {
       best_cost = cost;
+      best_ragged = ragged;
       best = sel;
     }
   }
