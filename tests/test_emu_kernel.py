@@ -455,3 +455,26 @@ def test_r2c_tile_order_paired_store(dims, prec):
     assert f"r2c[axis 0, N={dims[0]}, pow2" in desc, desc
     ref = orc.exec3d([C.R2C_D, C.CF_D, C.CF_D], dims, mo, mo, x)
     assert C.rel_l2(got, ref) < TOL[prec], desc
+
+
+def test_planner_avoids_ragged_tile_axes():
+    """Intermediate memory orders of equal primary cost: the planner takes
+    the one with the fewest passes whose tile axis extent is not a multiple
+    of 8.  R2C x / C2C y / DCT-I z with an odd spectrum x extent (25) runs
+    its DCT-I and C2C passes along the y axis (40); a grid whose axes are
+    all multiples of 8 keeps the input's order."""
+    types = [C.R2C_S, C.CF_S, C.r2r_id("DCT1", True, 4)]
+    dims = [48, 40, 33]
+    x = np.random.default_rng(7).uniform(-1, 1, orc.storage_shape(dims, [0, 1, 2]))
+    got, desc = emu.exec3d(types, dims, [0, 1, 2], [25, 40, 33], [0, 1, 2], x, 4)
+    assert "r2c[axis 0, N=48, mid pair, mo (012)->(102)" in desc, desc
+    assert "mo (102)->(102)" in desc and "mo (102)->(012)" in desc, desc
+    ref = orc.exec3d([C.R2C_D, C.CF_D, C.r2r_id("DCT1", True, 8)], dims, [0, 1, 2], [0, 1, 2], x)
+    assert C.rel_l2(got, ref) < TOL[4], desc
+    back, desc_b = emu.exec3d([C.C2R_S, C.CB_S, C.r2r_id("DCT1", True, 4)], [25, 40, 33], [0, 1, 2], dims,
+                              [0, 1, 2], got, 4)
+    assert "mo (012)->(102)" in desc_b and "mo (102)->(012)" in desc_b, desc_b
+    dims = [64, 64, 64]
+    x = np.random.default_rng(8).uniform(-1, 1, orc.storage_shape(dims, [2, 1, 0]))
+    got, desc = emu.exec3d([C.R2C_S, C.CF_S, C.CF_S], dims, [2, 1, 0], [33, 64, 64], [1, 2, 0], x, 4)
+    assert "mo (210)->(210)" in desc and "mo (210)->(120)" in desc, desc
