@@ -323,8 +323,10 @@ static bool setup_dct1_split(int prec, PassDesc &ps) {
   a.mid = 0;
   a.mid_prime = 0;
   a.pair = 0;
-  a.vec2 = 0;
-  a.vec2s = 0;
+  // paired lines on a tile-order side along a stride-1 e (whole tiles,
+  // strides keeping pairs aligned; the launcher checks the addresses)
+  a.vec2 = a.load_order == ORD_TILE && a.is_e == 1 && B >= 2 && a.ne % B == 0 && a.is_d % 2 == 0 && a.is_f % 2 == 0;
+  a.vec2s = a.store_order == ORD_TILE && a.os_e == 1 && B >= 2 && a.ne % B == 0 && a.os_d % 2 == 0 && a.os_f % 2 == 0;
   a.xcd_map = (a.load_order == ORD_TILE && B * esz < 128) || (a.store_order == ORD_TILE && B * esz < 128);
   a.tw_in_lds = 0;
   a.M = P;

@@ -788,6 +788,24 @@ P3E_HD P3E_INL void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, co
 // conj(e^{i pi q/2L}), so the pre-twiddle table has L/2 + 1 entries.
 // Region A (B lines of pitch lp) holds ext a, region B (B lines of pitch
 // lp2) holds V; tab = [twiddles of A (ntw_a) | of B (ntw_b) | pre-twiddles].
+// a and V of line j from the quad (x_q, x_{L-q}, x_{L+q}, x_{P-q})
+template <typename T, int P>
+P3E_HD P3E_INL void dct1_put_quad(const PassArgs &p, cx<T> *bufA, cx<T> *bufB, const cx<T> *pre, int j, int q,
+                                  cx<T> x0, cx<T> x1, cx<T> x2, cx<T> x3) {
+  constexpr int L = P / 2;
+  const cx<T> aq = add(x0, x3), al = add(x1, x2);
+  const cx<T> cq = sub(x0, x3), cl = sub(x1, x2);
+  cx<T> *A = bufA + j * p.lp;
+  A[P3E_PAD(q)] = aq;
+  A[P3E_PAD(L - q)] = al;
+  A[P3E_PAD(L + q)] = al;
+  if (q > 0) A[P3E_PAD(P - q)] = aq;
+  const cx<T> t = pre[q];
+  cx<T> *V = bufB + j * p.lp2;
+  V[P3E_PAD(q)] = mul(add(cq, rot(cl, -1)), t);
+  if (q > 0) V[P3E_PAD(L - q)] = mul(add(cq, rot(cl, 1)), conj(t));
+}
+
 template <typename T, int P, bool FULL>
 P3E_HD P3E_INL void dct1_load(const PassArgs &p, uint32_t f, int e0, int nb, cx<T> *bufA, cx<T> *bufB,
                               const cx<T> *pre, int tid, int nth) {
@@ -797,6 +815,37 @@ P3E_HD P3E_INL void dct1_load(const PassArgs &p, uint32_t f, int e0, int nb, cx<
   constexpr uint32_t CS = 2 * sizeof(T);
   const char *tin = reinterpret_cast<const char *>(p.in) + ((long long)e0 * p.is_e + (long long)f * p.is_f) * CS;
   const int isd = (int)p.is_d, ise = (int)p.is_e;
+  if (p.vec2) {
+    // tile order along a stride-1 e, whole tiles: the quads of lines
+    // j = 2 jp and j + 1 in 16-byte loads, B Q / 2 entries
+    constexpr int QP = (Q * P3E_FAST_E + 2 * (P + L) - 1) / (2 * (P + L));
+    const int lb = p.logB - 1, ptot = (p.B >> 1) * Q;
+    CxPair<T> y[QP][4];
+    int jq2[QP][2];
+#pragma unroll
+    for (int u = 0; u < QP; u++) {
+      const int i = tid + u * nth;
+      const bool ok = i < ptot;
+      const int j = 2 * (i & ((1 << lb) - 1)), q = i >> lb;
+      jq2[u][0] = ok ? j : -1;
+      jq2[u][1] = q;
+      const uint32_t o = ok ? (uint32_t)j : 0u, qq = ok ? (uint32_t)q : 0u;
+      y[u][0] = ldg<CxPair<T>>(tin, (o + qq * isd) * CS);
+      y[u][1] = ldg<CxPair<T>>(tin, (o + (L - qq) * isd) * CS);
+      y[u][2] = ldg<CxPair<T>>(tin, (o + (L + qq) * isd) * CS);
+      y[u][3] = ldg<CxPair<T>>(tin, (o + (P - qq) * isd) * CS);
+    }
+#pragma unroll
+    for (int u = 0; u < QP; u++) {
+      const int j = jq2[u][0], q = jq2[u][1];
+      if (j < 0) continue;
+      dct1_put_quad<T, P>(p, bufA, bufB, pre, j, q, mk(y[u][0].ar, y[u][0].ai), mk(y[u][1].ar, y[u][1].ai),
+                          mk(y[u][2].ar, y[u][2].ai), mk(y[u][3].ar, y[u][3].ai));
+      dct1_put_quad<T, P>(p, bufA, bufB, pre, j + 1, q, mk(y[u][0].br, y[u][0].bi), mk(y[u][1].br, y[u][1].bi),
+                          mk(y[u][2].br, y[u][2].bi), mk(y[u][3].br, y[u][3].bi));
+    }
+    return;
+  }
   const int tot = (p.load_order == ORD_LINE ? nb : p.B) * Q;
   cx<T> x[QN][4];
   int jq[QN][2];
@@ -825,17 +874,7 @@ P3E_HD P3E_INL void dct1_load(const PassArgs &p, uint32_t f, int e0, int nb, cx<
   for (int u = 0; u < QN; u++) {
     const int j = jq[u][0], q = jq[u][1];
     if (j < 0) continue;
-    const cx<T> aq = add(x[u][0], x[u][3]), al = add(x[u][1], x[u][2]);
-    const cx<T> cq = sub(x[u][0], x[u][3]), cl = sub(x[u][1], x[u][2]);
-    cx<T> *A = bufA + j * p.lp;
-    A[P3E_PAD(q)] = aq;
-    A[P3E_PAD(L - q)] = al;
-    A[P3E_PAD(L + q)] = al;
-    if (q > 0) A[P3E_PAD(P - q)] = aq;
-    const cx<T> t = pre[q];
-    cx<T> *V = bufB + j * p.lp2;
-    V[P3E_PAD(q)] = mul(add(cq, rot(cl, -1)), t);
-    if (q > 0) V[P3E_PAD(L - q)] = mul(add(cq, rot(cl, 1)), conj(t));
+    dct1_put_quad<T, P>(p, bufA, bufB, pre, j, q, x[u][0], x[u][1], x[u][2], x[u][3]);
   }
 }
 
@@ -846,6 +885,26 @@ P3E_HD P3E_INL void dct1_store(const PassArgs &p, uint32_t f, int e0, int nb, co
   constexpr uint32_t CS = 2 * sizeof(T);
   char *tout = reinterpret_cast<char *>(p.out) + ((long long)e0 * p.os_e + (long long)f * p.os_f) * CS;
   const int osd = (int)p.os_d, ose = (int)p.os_e;
+  if (p.vec2s) {
+    // tile order along a stride-1 e, whole tiles: output k of lines
+    // j = 2 jp and j + 1 in one 16-byte store
+    const int lb = p.logB - 1, ptot = (p.B >> 1) * (P + 1);
+#pragma unroll 4
+    for (int i = tid; i < ptot; i += nth) {
+      const int j = 2 * (i & ((1 << lb) - 1)), k = i >> lb;
+      const int vi = (k & 2) ? L - 1 - (k >> 2) : (k >> 2);
+      const cx<T> y0 = (k & 1) ? bufB[j * p.lp2 + P3E_PAD(vi)] : bufA[j * p.lp + P3E_PAD(k >> 1)];
+      const cx<T> y1 = (k & 1) ? bufB[(j + 1) * p.lp2 + P3E_PAD(vi)] : bufA[(j + 1) * p.lp + P3E_PAD(k >> 1)];
+      const cx<T> z0 = out_rule(p, y0, k, e0 + j, f), z1 = out_rule(p, y1, k, e0 + j + 1, f);
+      CxPair<T> v;
+      v.ar = z0.x;
+      v.ai = z0.y;
+      v.br = z1.x;
+      v.bi = z1.y;
+      stg(tout, (uint32_t)(k * osd + j) * CS, v);
+    }
+    return;
+  }
   const int tot = (p.store_order == ORD_LINE ? nb : p.B) * (P + 1);
 #pragma unroll 4
   for (int i = tid; i < tot; i += nth) {
