@@ -502,12 +502,14 @@ def main():
     torch.cuda.empty_cache()
     fp64 = None if args.no_fp64 else fp64_roundtrip(P, torch, n)
     mixed = None if args.no_mixed else mixed_bench(P, torch)
+    batch = None if args.no_batch else batch_bench(P, torch)
+    serving = None if args.no_serving else serving_bench(P, torch)
+    # last: 27 s of single-threaded host work, which would otherwise slow the
+    # host-side scheduling of the service measured after it
     cpu = None
     if not args.no_cpu_baseline:
         log(f"cpu baseline (oracle, {n}^3 FP64 round trip, one thread) ...")
         cpu = cpu_baseline(n)
-    batch = None if args.no_batch else batch_bench(P, torch)
-    serving = None if args.no_serving else serving_bench(P, torch)
     out = {
         "metric": METRIC,
         "value": round(value, 2),
