@@ -1,5 +1,5 @@
-"""Writes the golden fixtures under tests/golden/ (run once, in the build
-container, with numpy + scipy; the GPU box only reads the .npz files).
+"""Writes the golden fixtures under tests/golden/ (run once, where numpy and
+scipy are installed; the tests only read the .npz files).
 
 Fixtures:
   fftw_r2r_double.npz  FFTW-generated DCT/DST I-IV vectors (input a[i] = i,
