@@ -401,7 +401,7 @@ def main():
                          "known-size kernel that scales HBM byte counters under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs the 1x1 (single-GPU) transform path only; "
+        print(json.dumps({"error": "the library runs the 1x1 (single-GPU) transform path only; "
                                    "use --gpus 1"}), flush=True)
         sys.exit(2)
 
