@@ -294,6 +294,8 @@ static bool build_groups(BatchPlan *b) {
       g.tab0 = b->table.size();
       g.n = (int)kv.second.size();
       g.d_start = nullptr;
+      g.work = 0;
+      g.q = 0;
       std::vector<uint32_t> start(1, 0);
       unsigned long long tiles = 0;
       for (int i : kv.second) {
@@ -303,6 +305,7 @@ static bool build_groups(BatchPlan *b) {
         g.threads = std::max(g.threads, pd.threads);
         g.lds = std::max(g.lds, pd.lds_bytes);
         tiles += pd.nblocks;
+        g.work += (double)pd.in_bytes + (double)pd.out_bytes;
         if (tiles >= (1ull << 31)) {
           set_error("batch: too many tiles in one stage");
           return false;
@@ -322,6 +325,20 @@ static bool build_groups(BatchPlan *b) {
                b->desc.empty() ? "" : " | ", s, g.kind == 1 ? "pow2 M=" : "mid", g.kind == 1 ? g.M : 0, g.n,
                g.nblocks, g.threads, g.lds);
       b->desc += buf;
+    }
+    // The stage's launches, largest first, each on the stream with the least
+    // work so far (longest-processing-time first over BATCH_STREAMS
+    // streams): the device runs about that many kernels at once, and a
+    // stage ends with its last stream.
+    std::vector<BatchGroup> &gs = b->groups[s];
+    std::stable_sort(gs.begin(), gs.end(), [](const BatchGroup &x, const BatchGroup &y) { return x.work > y.work; });
+    double load[BATCH_STREAMS] = {0};
+    for (BatchGroup &g : gs) {
+      int q = 0;
+      for (int k = 1; k < BATCH_STREAMS; k++)
+        if (load[k] < load[q]) q = k;
+      g.q = q;
+      load[q] += g.work;
     }
   }
   return true;
@@ -446,11 +463,12 @@ BatchPlan *batch_create(int prec, int nitems, const OpSpec ops[3], const int ord
     return nullptr;
   }
   if (b->grouped) {
-    size_t most = 0;
-    for (const auto &gs : b->groups) most = std::max(most, gs.size());
+    int most = 0;
+    for (const auto &gs : b->groups)
+      for (const BatchGroup &g : gs) most = std::max(most, g.q + 1);
     b->fork_ev.assign(b->nstages, nullptr);
     for (int s = 0; s < b->nstages; s++) (void)hipEventCreateWithFlags(&b->fork_ev[s], hipEventDisableTiming);
-    for (size_t q = 0; q + 1 < most; q++) {
+    for (int q = 0; q + 1 < most; q++) {
       hipStream_t hs = nullptr;
       hipEvent_t he = nullptr;
       if (hipStreamCreateWithFlags(&hs, hipStreamNonBlocking) != hipSuccess ||
@@ -481,19 +499,26 @@ static int batch_launch(BatchPlan *b, hipStream_t st) {
   for (int s = 0; s < b->nstages; s++) {
     if (b->grouped) {
       const std::vector<BatchGroup> &gs = b->groups[s];
-      // the groups of a stage are independent: run concurrently, one
-      // group's tail overlaps the others (1.99 -> 1.80 ms for 256 items of
-      // 32..128 points, six groups a stage)
-      const int nside = std::min((int)gs.size() - 1, (int)b->side.size());
-      if (nside > 0) {
-        BCHK(hipEventRecord(b->fork_ev[s], st));
-        for (int q = 0; q < nside; q++) BCHK(hipStreamWaitEvent(b->side[q], b->fork_ev[s], 0));
+      // the groups of a stage are independent: they run concurrently on the
+      // exec stream and side streams forked from it (their stream planned in
+      // build_groups), so one group's tail overlaps the others (1.99 -> 1.80
+      // ms for 256 items of 32..128 points, six groups a stage)
+      bool used[BATCH_STREAMS] = {false};
+      for (const BatchGroup &g : gs) used[g.q] = true;
+      for (int q = 1; q < BATCH_STREAMS; q++)
+        if (used[q] && q - 1 >= (int)b->side.size()) {
+          set_error("batch: group planned on a missing side stream");
+          return -1;
+        }
+      bool forked = false;
+      for (int q = 1; q < BATCH_STREAMS; q++) {
+        if (!used[q]) continue;
+        if (!forked) BCHK(hipEventRecord(b->fork_ev[s], st));
+        forked = true;
+        BCHK(hipStreamWaitEvent(b->side[q - 1], b->fork_ev[s], 0));
       }
-      for (size_t gi = 0; gi < gs.size(); gi++) {
-        const BatchGroup &g = gs[gi];
-        // group gi on side stream gi - 1 (the exec stream takes group 0 and
-        // any beyond the side streams)
-        hipStream_t gst = (gi >= 1 && (int)gi <= nside) ? b->side[gi - 1] : st;
+      for (const BatchGroup &g : gs) {
+        hipStream_t gst = g.q == 0 ? st : b->side[g.q - 1];
         const int rc = p3e_launch_group(g.kind, g.M, b->d_table + g.tab0, g.d_start, g.n, b->prec, g.lds, g.threads,
                                         g.nblocks, gst);
         if (rc) {
@@ -501,9 +526,10 @@ static int batch_launch(BatchPlan *b, hipStream_t st) {
           return -1;
         }
       }
-      for (int q = 0; q < nside; q++) {
-        BCHK(hipEventRecord(b->join_ev[q], b->side[q]));
-        BCHK(hipStreamWaitEvent(st, b->join_ev[q], 0));
+      for (int q = 1; q < BATCH_STREAMS; q++) {
+        if (!used[q]) continue;
+        BCHK(hipEventRecord(b->join_ev[q - 1], b->side[q - 1]));
+        BCHK(hipStreamWaitEvent(st, b->join_ev[q - 1], 0));
       }
     } else {
       size_t lds = 0;

@@ -20,7 +20,13 @@ struct BatchGroup {
   size_t tab0;
   int n;
   uint32_t *d_start;       // n + 1 workgroup offsets (device)
+  double work;             // bytes the n items' passes read and write
+  int q;                   // stream: 0 the exec stream, q > 0 side stream q - 1
 };
+
+// Streams a batch stage's grouped launches are spread over (the device's
+// default number of hardware queues per process).
+static const int BATCH_STREAMS = 4;
 
 struct BatchPlan {
   int prec;
