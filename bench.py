@@ -323,6 +323,39 @@ def fp64_roundtrip(P, torch, n, steps=10):
     return res
 
 
+def small_bench(P, torch, steps=50):
+    """64^3 FP64 R2C+C2R round trip, memory order (0,1,2) in, (1,2,0) out
+    (the samples' small configuration): launch-latency bound, so reported
+    as time per round trip and per pass launch, not as a share of peak."""
+    n = 64
+    pg = P.init_proc_grid((1, 1, 1))
+    g_real = P.init_data_grid((n, n, n), -1, pg, (0, 1, 2), (0, 1, 2))
+    g_spec = P.init_data_grid((n // 2 + 1, n, n), 0, pg, (0, 1, 2), (1, 2, 0))
+    pf = P.plan_3Dtrans(g_real, g_spec, P.init_3Dtype((P.type_id("R2CFFT_D"), P.type_id("CFFT_FORWARD_D"),
+                                                        P.type_id("CFFT_FORWARD_D"))))
+    pb = P.plan_3Dtrans(g_spec, g_real, P.init_3Dtype((P.type_id("C2RFFT_D"), P.type_id("CFFT_BACKWARD_D"),
+                                                        P.type_id("CFFT_BACKWARD_D"))))
+    f = torch.rand((n, n, n), device="cuda", dtype=torch.float64, generator=torch.Generator(device="cuda").manual_seed(1))
+    spec = torch.empty((n, n // 2 + 1, n), dtype=torch.complex128, device="cuda")
+    back = torch.empty_like(f)
+    st = P.stream()
+    torch.cuda.synchronize()
+    for i in range(steps + 3):
+        if i == 3:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+        P.exec_async(pf, f.data_ptr(), spec.data_ptr(), -1, 0, st, prec=8)
+        P.exec_async(pb, spec.data_ptr(), back.data_ptr(), -1, 0, st, prec=8)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / steps
+    err = (torch.linalg.vector_norm(back / n ** 3 - f) / torch.linalg.vector_norm(f)).item()
+    npass = P.npasses(pf) + P.npasses(pb)
+    log(f"64^3 FP64 round trip {t * 1e6:.1f} us ({npass} pass launches), rel L2 {err:.2e}")
+    return {"workload": "64^3 R2C+C2R FP64 round trip, mem order (0,1,2) -> (1,2,0)",
+            "us_per_roundtrip": round(t * 1e6, 2), "pass_launches": npass,
+            "us_per_pass": round(t * 1e6 / npass, 2), "roundtrip_rel_l2": err}
+
+
 def mixed_bench(P, torch, steps=10):
     """768 x 640 x 513 real grid, FP32: R2C x, C2C y, DCT1_COMPLEX z forward
     and back (memory order (0,1,2) on both sides), field of Chebyshev series
@@ -484,6 +517,7 @@ def main():
     # dominant kernel: pass_kernel_pow2<float, 512>, the four C2C passes of a
     # step (forward y, z; backward y, z), each reading and writing V_c
     vc = 2 * 4 * (n // 2 + 1) * n * n
+    vr = 4 * n ** 3
     c2c_ms = [avg_f[1], avg_f[2], avg_b[0], avg_b[1]]
     c2c_avg = sum(c2c_ms) / 4
     achieved = 2 * vc / (c2c_avg * 1e-3) / 1e9
@@ -501,6 +535,7 @@ def main():
     del f, spec, back
     torch.cuda.empty_cache()
     fp64 = None if args.no_fp64 else fp64_roundtrip(P, torch, n)
+    small = None if args.no_fp64 else small_bench(P, torch)
     mixed = None if args.no_mixed else mixed_bench(P, torch)
     batch = None if args.no_batch else batch_bench(P, torch)
     serving = None if args.no_serving else serving_bench(P, torch)
@@ -524,6 +559,10 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "pct_hbm_peak": round(100.0 * value / HBM_PEAK_GBS, 2),
+        # compulsory bytes: one read of the input and one write of the
+        # output per direction, 2 (V_r + V_c); reported beside the metric,
+        # never in its place (a multi-pass algorithm moves B_alg)
+        "compulsory": {"bytes": 2 * (vr + vc), "GBps": round(2 * (vr + vc) / t_step / 1e9, 1)},
         "roundtrip_rel_l2": err,
         "config": {"workload": f"{n}^3 R2C+C2R FP32 round trip, in mem order (2,1,0), spectrum (1,2,0), "
                                "sin(x)sin(2y)sin(3z)+N(0,1e-3)",
@@ -538,6 +577,7 @@ def main():
                                f"steps right after the timed ones ({t_ev * 1e3:.4f} ms a step with the events)"},
         "cpu_baseline": cpu,
         "fp64_roundtrip": fp64,
+        "small_fp64_64cubed": small,
         "mixed": mixed,
         "batch": batch,
         "serving": serving,
