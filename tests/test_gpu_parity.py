@@ -427,3 +427,56 @@ def test_bluestein_above_47(prec):
     log("case: bluestein 53 59 61", prec, desc)
     assert desc.count("bluestein") == 3, desc
     assert C.rel_l2(got, orc.exec3d(types, dims, [0, 1, 2], [0, 1, 2], x)) < TOL[prec]
+
+
+# ------------------------------------------------- compile-time lengths
+
+FIXED_ODD = [48, 96, 192, 384, 768, 80, 160, 320, 640, 120]
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+@pytest.mark.parametrize("M", FIXED_ODD)
+def test_fixed_length_odd_kernels_vs_oracle(M, prec):
+    """Compile-time lengths 3, 5 and 15 times a power of two on the GPU:
+    C2C along a stride-1 axis (line order) and along a strided axis (tile
+    order with a partial tile), and R2C of length 2M (line order, and tile
+    order with paired two-line stores), against the oracle."""
+    cf = C.CF_D if prec == 8 else C.CF_S
+    for dims, mo1, mo2 in (([M, 3, 2], [0, 1, 2], [0, 1, 2]), ([5, M, 3], [0, 1, 2], [2, 0, 1])):
+        log("case: fixed c2c", M, prec, dims, mo1, mo2)
+        rng = np.random.default_rng(M)
+        x = rng.standard_normal(orc.storage_shape(dims, mo1)) + 1j * rng.standard_normal(orc.storage_shape(dims, mo1))
+        got, pl = gpu_3d([cf] * 3, dims, mo1, mo2, cast(x, prec))
+        assert f"N={M}, pow2" in P.describe(pl.h), P.describe(pl.h)
+        ref = orc.exec3d([C.CF_D] * 3, dims, mo1, mo2, wide(cast(x, prec)))
+        assert C.rel_l2(got, ref) < TOL[prec], P.describe(pl.h)
+    r2c = C.R2C_D if prec == 8 else C.R2C_S
+    for dims, mo in (([2 * M, 3, 4], [0, 1, 2]), ([2 * M, 2, 16], [2, 1, 0])):
+        log("case: fixed r2c", 2 * M, prec, dims, mo)
+        x = np.random.default_rng(2 * M).uniform(-1, 1, orc.storage_shape(dims, mo))
+        got, pl = gpu_3d([r2c, cf, cf], dims, mo, mo, cast(x, prec))
+        assert f"r2c[axis 0, N={2 * M}, pow2" in P.describe(pl.h), P.describe(pl.h)
+        ref = orc.exec3d([C.R2C_D, C.CF_D, C.CF_D], dims, mo, mo, wide(cast(x, prec)))
+        assert C.rel_l2(got, ref) < TOL[prec], P.describe(pl.h)
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+@pytest.mark.parametrize("n", [129, 257, 513, 1025])
+def test_dct1_complex_split_vs_oracle(n, prec):
+    """Complex DCT-I of n points in the split form on the GPU (a (n-1)-point
+    FFT and an (n-1)/2-point inverse FFT per line): tile order with paired
+    lines (stride-1 tile axis, whole tiles), tile order with a partial tile,
+    and line order, against the oracle."""
+    tid = C.r2r_id("DCT1", True, prec)
+    empty = C.EMPTY_DC if prec == 8 else C.EMPTY_SC
+    for dims, mo1, mo2 in (([16, 3, n], [0, 1, 2], [0, 1, 2]), ([7, 3, n], [0, 1, 2], [2, 0, 1]),
+                           ([n, 5, 3], [0, 1, 2], [0, 1, 2])):
+        log("case: dct1 split", n, prec, dims, mo1, mo2)
+        ax = dims.index(n)
+        types = [empty if a != ax else tid for a in range(3)]
+        rng = np.random.default_rng(n + prec)
+        x = rng.standard_normal(orc.storage_shape(dims, mo1)) + 1j * rng.standard_normal(orc.storage_shape(dims, mo1))
+        got, pl = gpu_3d(types, dims, mo1, mo2, cast(x, prec))
+        assert "pow2 split" in P.describe(pl.h), P.describe(pl.h)
+        ref = orc.exec_1d(C.r2r_id("DCT1", True, 8), ax, dims, mo1, dims, mo2, wide(cast(x, prec)))
+        assert C.rel_l2(got, ref) < TOL[prec], P.describe(pl.h)
