@@ -199,3 +199,23 @@ def test_batch_item_alone_equals_item_in_mix():
     for i in (0, 7, 23):
         alone, _ = run_batch([C.CF_S] * 3, [dims[i]], [dims[i]], [0, 1, 2], [0, 1, 2], [xs[i]], True, 4)
         assert np.array_equal(alone[0], mix[i]), f"item {i} {dims[i]}"
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+def test_batch_mixed_type_dct1_vs_oracle(prec):
+    """Batches of the mixed-type transform (R2C x, C2C y, DCT1_COMPLEX z) of
+    different sizes: z lengths with the split DCT-I (129, 257, 513) and
+    without it (65, 97) share grouped launches with C2C passes of the same
+    kernel length; each item against the FP64 oracle."""
+    dims = [[48, 40, 129], [64, 32, 257], [32, 24, 513], [40, 48, 65], [96, 16, 97], [24, 40, 257]]
+    log("case: batch mixed-type dct1 prec", prec)
+    xs = batch_inputs(dims, 9, prec, real=True)
+    d1 = C.r2r_id("DCT1", True, prec)
+    t = [C.R2C_S, C.CF_S, d1] if prec == 4 else [C.R2C_D, C.CF_D, d1]
+    odims = [C.out_dims(t, d) for d in dims]
+    got, desc = run_batch(t, dims, odims, [0, 1, 2], [0, 1, 2], xs, True, prec)
+    log("  ", desc)
+    ref = oracle_all([C.R2C_D, C.CF_D, C.r2r_id("DCT1", True, 8)], dims, [0, 1, 2], [0, 1, 2], xs)
+    errs = [C.rel_l2(g, r) for g, r in zip(got, ref)]
+    log("  max rel L2", max(errs))
+    assert max(errs) < TOL[prec]
