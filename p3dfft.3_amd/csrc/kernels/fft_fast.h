@@ -139,6 +139,13 @@ P3E_HD P3E_INL void fft_fixed(Run &run, cx<T> *buf, int lp, const cx<T> *tw, int
   else if constexpr (M == 160) fft_seq<T, M, 1, S, Run, 8, 4, 5>(run, buf, lp, tw, B);
   else if constexpr (M == 320) fft_seq<T, M, 1, S, Run, 16, 4, 5>(run, buf, lp, tw, B);
   else if constexpr (M == 120) fft_seq<T, M, 1, S, Run, 8, 5, 3>(run, buf, lp, tw, B);
+  else if constexpr (M == 60) fft_seq<T, M, 1, S, Run, 4, 5, 3>(run, buf, lp, tw, B);
+  else if constexpr (M == 72) fft_seq<T, M, 1, S, Run, 8, 9>(run, buf, lp, tw, B);
+  else if constexpr (M == 144) fft_seq<T, M, 1, S, Run, 16, 9>(run, buf, lp, tw, B);
+  else if constexpr (M == 112) fft_seq<T, M, 1, S, Run, 16, 7>(run, buf, lp, tw, B);
+  else if constexpr (M == 224) fft_seq<T, M, 1, S, Run, 8, 4, 7>(run, buf, lp, tw, B);
+  else if constexpr (M == 240) fft_seq<T, M, 1, S, Run, 16, 5, 3>(run, buf, lp, tw, B);
+  else if constexpr (M == 36) fft_seq<T, M, 1, S, Run, 4, 9>(run, buf, lp, tw, B);
   else fft_seq<T, M, 1, S, Run, 16, 8, 5>(run, buf, lp, tw, B);  // 640
 }
 

@@ -149,7 +149,9 @@ static inline void drop_unaligned_pairs(PassArgs &a, int prec) {
 // Radix sequence of the compile-time-length path for each complex length M
 // (the kernel's compile-time sequences in fft_fast.h are the same); 0 if M
 // is not supported.  Powers of two from 16 to 4096; 3 times 16 to 256, 5
-// times 16 to 128 and 120 = 8 5 3, with the odd radices last.
+// times 16 to 128, 7 times 16 and 32, 9 times 4 to 16, and 15 times 4, 8
+// and 16, with the odd radices last (radices 7 and 9 run as direct odd
+// DFTs, bf_prime).
 static inline int fast_radices(int M, int *rad) {
   switch (M) {
     case 48: rad[0] = 16; rad[1] = 3; return 2;
@@ -162,6 +164,13 @@ static inline int fast_radices(int M, int *rad) {
     case 320: rad[0] = 16; rad[1] = 4; rad[2] = 5; return 3;
     case 640: rad[0] = 16; rad[1] = 8; rad[2] = 5; return 3;
     case 120: rad[0] = 8; rad[1] = 5; rad[2] = 3; return 3;
+    case 60: rad[0] = 4; rad[1] = 5; rad[2] = 3; return 3;
+    case 72: rad[0] = 8; rad[1] = 9; return 2;
+    case 144: rad[0] = 16; rad[1] = 9; return 2;
+    case 112: rad[0] = 16; rad[1] = 7; return 2;
+    case 224: rad[0] = 8; rad[1] = 4; rad[2] = 7; return 3;
+    case 240: rad[0] = 16; rad[1] = 5; rad[2] = 3; return 3;
+    case 36: rad[0] = 4; rad[1] = 9; return 2;
 
     case 16: rad[0] = 16; return 1;
     case 32: rad[0] = 8; rad[1] = 4; return 2;

@@ -173,6 +173,13 @@ template <typename T> static int run_pass(const PassDesc &pd, const void *in, vo
       case 320: return run_pow2<T, 320>(pd, a);
       case 640: return run_pow2<T, 640>(pd, a);
       case 120: return run_pow2<T, 120>(pd, a);
+      case 60: return run_pow2<T, 60>(pd, a);
+      case 72: return run_pow2<T, 72>(pd, a);
+      case 144: return run_pow2<T, 144>(pd, a);
+      case 112: return run_pow2<T, 112>(pd, a);
+      case 224: return run_pow2<T, 224>(pd, a);
+      case 240: return run_pow2<T, 240>(pd, a);
+      case 36: return run_pow2<T, 36>(pd, a);
     }
     return -1;
   }

@@ -353,7 +353,7 @@ def test_padded_intermediate_arrays(prec, mo1, mo2):
 
 
 @pytest.mark.parametrize("prec", [8, 4])
-@pytest.mark.parametrize("M", [48, 96, 192, 384, 768, 80, 160, 320, 640, 120])
+@pytest.mark.parametrize("M", [48, 96, 192, 384, 768, 80, 160, 320, 640, 120, 60, 72, 144, 112, 224, 240, 36])
 def test_fixed_length_odd_radix(M, prec):
     """Compile-time lengths 3 and 5 times a power of two (the odd radix pass
     has ceil(16 / R) butterfly slots per thread): C2C along a stride-1 axis

@@ -431,7 +431,7 @@ def test_bluestein_above_47(prec):
 
 # ------------------------------------------------- compile-time lengths
 
-FIXED_ODD = [48, 96, 192, 384, 768, 80, 160, 320, 640, 120]
+FIXED_ODD = [48, 96, 192, 384, 768, 80, 160, 320, 640, 120, 60, 72, 144, 112, 224, 240, 36]
 
 
 @pytest.mark.parametrize("prec", [4, 8])
