@@ -277,6 +277,26 @@ template <typename V> P3E_HD P3E_INL V ldg(const char *base, uint32_t off) {
 template <typename V> P3E_HD P3E_INL void stg(char *base, uint32_t off, const V &v) {
   *reinterpret_cast<V *>(base + off) = v;
 }
+// Non-temporal store (streaming: the line is not kept in the caches), for
+// line-order stores, which write whole lines; tile-order stores write
+// partial cache lines that the next tile along e completes, and keep stg.
+template <typename V> P3E_HD P3E_INL void stg_nt(char *base, uint32_t off, const V &v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (sizeof(V) == 4) {
+    __builtin_nontemporal_store(*reinterpret_cast<const unsigned *>(&v), reinterpret_cast<unsigned *>(base + off));
+  } else if constexpr (sizeof(V) == 8) {
+    __builtin_nontemporal_store(*reinterpret_cast<const unsigned long long *>(&v),
+                                reinterpret_cast<unsigned long long *>(base + off));
+  } else if constexpr (sizeof(V) == 16) {
+    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+    __builtin_nontemporal_store(*reinterpret_cast<const u4 *>(&v), reinterpret_cast<u4 *>(base + off));
+  } else {
+    *reinterpret_cast<V *>(base + off) = v;
+  }
+#else
+  *reinterpret_cast<V *>(base + off) = v;
+#endif
+}
 
 // FULL: the tile has all B lines (nb == B); every entry a thread owns is
 // then inside the tile and the accesses need no bounds checks.
@@ -655,7 +675,7 @@ P3E_HD P3E_INL void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, co
       RealPair<T> v;
       v.a = p.has_scale ? z.x * (T)p.scale : z.x;
       v.b = p.has_scale ? z.y * (T)p.scale : z.y;
-      stg(tout, (uint32_t)(2 * m + j * ose) * RS, v);
+      stg_nt(tout, (uint32_t)(2 * m + j * ose) * RS, v);
     }
   } else if (p.vec2s && p.cls == CLS_C2R) {
     // C2R, tile order, e stride-1 and whole tiles (the planner's condition):
@@ -716,8 +736,8 @@ P3E_HD P3E_INL void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, co
       cx<T> xk, xm;
       r2c_split2<T, M>(buf + j * lp, wN, k, &xk, &xm);
       const uint32_t lo = (uint32_t)(j * ose);
-      stg(tout, (lo + (uint32_t)k) * CS, out_rule(p, xk, k, e0 + j, f));
-      if (2 * k != M) stg(tout, (lo + (uint32_t)(M - k)) * CS, out_rule(p, xm, M - k, e0 + j, f));
+      stg_nt(tout, (lo + (uint32_t)k) * CS, out_rule(p, xk, k, e0 + j, f));
+      if (2 * k != M) stg_nt(tout, (lo + (uint32_t)(M - k)) * CS, out_rule(p, xm, M - k, e0 + j, f));
     }
   } else if (p.vec2s && p.store_order == ORD_LINE) {
     // C2C, line order (d stride-1): elements k = 2 kp and k + 1 of line j
@@ -733,7 +753,7 @@ P3E_HD P3E_INL void fast_store(const PassArgs &p, uint32_t f, int e0, int nb, co
       v.ai = y0.y;
       v.br = y1.x;
       v.bi = y1.y;
-      stg(tout, (uint32_t)(k + j * ose) * CS, v);
+      stg_nt(tout, (uint32_t)(k + j * ose) * CS, v);
     }
   } else if (p.vec2s) {
     // C2C and R2C, tile order (e stride-1, whole tiles): element k of lines
